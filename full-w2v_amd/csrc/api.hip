@@ -1,0 +1,667 @@
+// C-ABI entry points of libw2v (declared in include/w2v.h): context, device memory,
+// submission streams, staging, device-resident corpus and counters.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "w2v.h"
+
+namespace w2v {
+hipError_t launch_plain(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st);
+hipError_t occupancy_plain(int P, int waves_per_block, size_t lds, int* blocks_per_cu);
+size_t plain_lds_per_wave(int W, int P);
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
+size_t compact_temp_bytes(int64_t n);
+hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t sb, int64_t se,
+                          const float* subs_p, uint64_t mixed_seed, int32_t epoch,
+                          int32_t* flags, int32_t* pos, int32_t* kept, int64_t* kept_off,
+                          void* temp, size_t temp_bytes, int64_t tok_begin, int64_t n,
+                          hipStream_t st);
+hipError_t launch_heldout_loss(const float* syn0, const float* syn1, int P, int W, int K,
+                               const int32_t* tokens, const int64_t* off, int64_t n_sent,
+                               const int32_t* unigram, int64_t n_unigram, uint64_t mixed_seed,
+                               double* acc, unsigned long long* pairs, int num_cu, hipStream_t st);
+}  // namespace w2v
+
+using namespace w2v;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct StreamState {
+  hipStream_t st = nullptr;
+  unsigned int* queue = nullptr;
+  // staging for host batches: one batch in flight per stream
+  HostBuf h_tok, h_neg, h_len, h_alpha, h_off;
+  DevBuf d_tok, d_neg, d_len, d_alpha, d_off;
+  // device-corpus step buffers
+  DevBuf flags, pos, kept, kept_off, temp;
+  // timing
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0.0;
+  uint64_t launches = 0;
+};
+
+}  // namespace
+
+struct w2v_ctx {
+  w2v_params p{};
+  int P = 0;  // padded row length (elements)
+  int num_cu = 0;
+  int waves_per_block = 4;
+  int blocks = 0;  // persistent grid of the update kernel
+  float* syn0 = nullptr;
+  float* syn1 = nullptr;
+  float* dsyn0 = nullptr;
+  float* dsyn1 = nullptr;
+  int delta_mode = 0;
+  float* d_exp = nullptr;
+  float* d_nexp = nullptr;
+  unsigned long long* d_counters = nullptr;  // [0] words trained, [1] sentences
+  std::vector<StreamState> ss;
+  // device sampling
+  int32_t* d_unigram = nullptr;
+  int64_t n_unigram = 0;
+  float* d_subs = nullptr;
+  int32_t* d_corpus = nullptr;
+  int64_t n_tokens = 0;
+  int64_t* d_off = nullptr;
+  std::vector<int64_t> h_off;
+  int64_t n_sent = 0;
+  float alpha0 = 0.025f;
+  int32_t epochs = 1;
+  uint64_t train_words = 0;
+  uint64_t words_read = 0;
+  uint64_t launches = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(w2v_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  else g_create_error = buf;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((ctx), W2V_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+int ensure_dev(w2v_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.cap >= bytes) return W2V_OK;
+  if (b.p) HIP_TRY(c, hipFree(b.p));
+  b.p = nullptr;
+  b.cap = 0;
+  size_t want = bytes < 256 ? 256 : bytes;
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, W2V_ERR_NOMEM, "device allocation of %zu bytes failed", want);
+  }
+  b.cap = want;
+  return W2V_OK;
+}
+
+int ensure_host(w2v_ctx* c, HostBuf& b, size_t bytes) {
+  if (b.cap >= bytes) return W2V_OK;
+  if (b.p) HIP_TRY(c, hipHostFree(b.p));
+  b.p = nullptr;
+  b.cap = 0;
+  size_t want = bytes < 256 ? 256 : bytes;
+  if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, W2V_ERR_NOMEM, "pinned allocation of %zu bytes failed", want);
+  }
+  b.cap = want;
+  return W2V_OK;
+}
+
+// expTable/nexpTable exactly as generate_expTable_h.cpp:24-74 computes them (float
+// arithmetic, MAX_EXP 6, ends clamped), size 1000.
+void make_tables(float* e, float* ne) {
+  auto val = [](int i) {
+    float v = expf((i / (float)kExpTableSize * 2 - 1) * 6);
+    return v / (v + 1);
+  };
+  for (int i = 0; i < kExpTableSize; i++)
+    e[i] = (i == 0) ? 0.0f : (i + 1 == kExpTableSize) ? 1.0f : val(i);
+  for (int i = kExpTableSize - 1; i >= 0; i--) {
+    const int j = kExpTableSize - 1 - i;
+    ne[j] = (i == kExpTableSize - 1) ? 1.0f : (i == 0) ? 0.0f : val(i);
+  }
+}
+
+int check_stream(w2v_ctx* c, int32_t stream) {
+  if (!c) return W2V_ERR_ARG;
+  if (stream < 0 || stream >= (int32_t)c->ss.size())
+    return fail(c, W2V_ERR_ARG, "stream index %d out of range [0,%zu)", stream, c->ss.size());
+  return W2V_OK;
+}
+
+int drain_timing(w2v_ctx* c, StreamState& s) {
+  for (auto& pr : s.pending) {
+    HIP_TRY(c, hipEventSynchronize(pr.second));
+    float ms = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, pr.first, pr.second));
+    s.ms += ms;
+    HIP_TRY(c, hipEventDestroy(pr.first));
+    HIP_TRY(c, hipEventDestroy(pr.second));
+  }
+  s.pending.clear();
+  return W2V_OK;
+}
+
+UpdateArgs base_args(w2v_ctx* c) {
+  UpdateArgs a{};
+  a.syn0 = c->syn0;
+  a.syn1 = c->syn1;
+  if (c->delta_mode) {
+    a.out0 = c->dsyn0;
+    a.out1 = c->dsyn1;
+    a.rd0 = c->dsyn0;
+    a.rd1 = c->dsyn1;
+  } else {
+    a.out0 = c->syn0;
+    a.out1 = c->syn1;
+  }
+  a.d = c->p.layer_size;
+  a.P = c->P;
+  a.W = c->p.window;
+  a.K = c->p.negative;
+  a.msl = c->p.max_sentence_length;
+  a.sig_exact = c->p.sigmoid == W2V_SIGMOID_EXACT;
+  a.fence = c->p.mode == W2V_MODE_DETERMINISTIC;
+  a.exp_t = c->d_exp;
+  a.nexp_t = c->d_nexp;
+  a.counters = c->d_counters;
+  a.mixed_seed = mix64(c->p.seed);
+  return a;
+}
+
+// Launches the update kernel for a prepared argument block on one stream.
+int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a) {
+  if (a.n_sent <= 0) return W2V_OK;
+  if (a.n_sent >= (int64_t)1 << 31)
+    return fail(c, W2V_ERR_ARG, "at most 2^31-1 sentences per launch (got %lld)", (long long)a.n_sent);
+  a.queue = s.queue;
+  HIP_TRY(c, launch_zero_u32(s.queue, 1, s.st));
+  int wpb = c->waves_per_block, blocks = c->blocks;
+  if (c->p.mode == W2V_MODE_DETERMINISTIC) {
+    wpb = 1;
+    blocks = 1;
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  HIP_TRY(c, hipEventRecord(e0, s.st));
+  HIP_TRY(c, launch_plain(a, wpb, blocks, s.st));
+  HIP_TRY(c, hipEventRecord(e1, s.st));
+  s.pending.emplace_back(e0, e1);
+  s.launches++;
+  c->launches++;
+  if (s.pending.size() > 64) {
+    // keep the event list bounded: drain the oldest completed half
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> keep(s.pending.begin() + 32, s.pending.end());
+    s.pending.resize(32);
+    int rc = drain_timing(c, s);
+    s.pending = keep;
+    if (rc) return rc;
+  }
+  return W2V_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int w2v_create(const w2v_params* params, w2v_ctx** out) {
+  if (!params || !out) return fail(nullptr, W2V_ERR_ARG, "null argument");
+  *out = nullptr;
+  const w2v_params& p = *params;
+  if (p.vocab_size <= 0) return fail(nullptr, W2V_ERR_ARG, "vocab_size must be positive");
+  if (p.layer_size < 1 || p.layer_size > 1024)
+    return fail(nullptr, W2V_ERR_UNSUPPORTED, "layer_size %d outside 1..1024", p.layer_size);
+  if (p.window < 1 || p.window > 31)
+    return fail(nullptr, W2V_ERR_UNSUPPORTED, "window %d outside 1..31", p.window);
+  if (p.negative < 0 || p.negative > 64)
+    return fail(nullptr, W2V_ERR_UNSUPPORTED, "negative %d outside 0..64", p.negative);
+  if (p.max_sentence_length < 1) return fail(nullptr, W2V_ERR_ARG, "max_sentence_length must be >= 1");
+  if (p.streams < 1 || p.streams > 64) return fail(nullptr, W2V_ERR_ARG, "streams must be in 1..64");
+  if (p.storage != W2V_STORAGE_FP32)
+    return fail(nullptr, W2V_ERR_UNSUPPORTED, "storage %d not available in this build", p.storage);
+  if (p.kernel != W2V_KERNEL_PLAIN)
+    return fail(nullptr, W2V_ERR_UNSUPPORTED, "kernel %d not available in this build", p.kernel);
+  if (p.sigmoid != W2V_SIGMOID_TABLE && p.sigmoid != W2V_SIGMOID_EXACT)
+    return fail(nullptr, W2V_ERR_ARG, "unknown sigmoid mode %d", p.sigmoid);
+  if (p.mode != W2V_MODE_HOGWILD && p.mode != W2V_MODE_DETERMINISTIC)
+    return fail(nullptr, W2V_ERR_ARG, "unknown mode %d", p.mode);
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(nullptr, W2V_ERR_HIP, "no HIP device available");
+  if (p.device < 0 || p.device >= ndev)
+    return fail(nullptr, W2V_ERR_ARG, "device %d out of range (%d devices)", p.device, ndev);
+
+  w2v_ctx* c = new w2v_ctx();
+  c->p = p;
+  c->P = (p.layer_size + 63) / 64 * 64;
+  int rc = W2V_OK;
+  auto bail = [&](int code) {
+    g_create_error = c->err;
+    w2v_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(p.device) != hipSuccess) return bail(fail(c, W2V_ERR_HIP, "hipSetDevice failed"));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, p.device) != hipSuccess)
+    return bail(fail(c, W2V_ERR_HIP, "hipGetDeviceProperties failed"));
+  c->num_cu = prop.multiProcessorCount;
+
+  const size_t lds_wave = plain_lds_per_wave(p.window, c->P);
+  const size_t lds_max = 160 * 1024;
+  if (lds_wave > lds_max)
+    return bail(fail(c, W2V_ERR_UNSUPPORTED,
+                     "window ring of %zu bytes exceeds the 160 KiB LDS of a CU (d=%d, W=%d)",
+                     lds_wave, p.layer_size, p.window));
+  int wpb = 4;
+  while (wpb > 1 && lds_wave * wpb > lds_max) wpb--;
+  c->waves_per_block = wpb;
+  int bpc = 0;
+  if (occupancy_plain(c->P, wpb, lds_wave * wpb, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+  c->blocks = bpc * c->num_cu;
+
+  const size_t rows = (size_t)p.vocab_size * c->P * sizeof(float);
+  if (hipMalloc(&c->syn0, rows) != hipSuccess || hipMalloc(&c->syn1, rows) != hipSuccess)
+    return bail(fail(c, W2V_ERR_NOMEM, "model allocation of 2 x %zu bytes failed", rows));
+  if (hipMemset(c->syn0, 0, rows) != hipSuccess || hipMemset(c->syn1, 0, rows) != hipSuccess)
+    return bail(fail(c, W2V_ERR_HIP, "model clear failed"));
+  float e[kExpTableSize], ne[kExpTableSize];
+  make_tables(e, ne);
+  if (hipMalloc(&c->d_exp, sizeof(e)) != hipSuccess || hipMalloc(&c->d_nexp, sizeof(ne)) != hipSuccess ||
+      hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess)
+    return bail(fail(c, W2V_ERR_NOMEM, "table allocation failed"));
+  if (hipMemcpy(c->d_exp, e, sizeof(e), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_nexp, ne, sizeof(ne), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(c->d_counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess)
+    return bail(fail(c, W2V_ERR_HIP, "table upload failed"));
+  c->ss.resize(p.streams);
+  for (auto& s : c->ss) {
+    if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(c, W2V_ERR_HIP, "stream creation failed"));
+    if (hipMalloc(&s.queue, 256) != hipSuccess) return bail(fail(c, W2V_ERR_NOMEM, "queue allocation failed"));
+  }
+  (void)rc;
+  *out = c;
+  return W2V_OK;
+}
+
+int w2v_destroy(w2v_ctx* c) {
+  if (!c) return W2V_OK;
+  if (c->p.device >= 0) hipSetDevice(c->p.device);
+  for (auto& s : c->ss) {
+    if (s.st) hipStreamSynchronize(s.st);
+    for (auto& pr : s.pending) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+    for (HostBuf* h : {&s.h_tok, &s.h_neg, &s.h_len, &s.h_alpha, &s.h_off})
+      if (h->p) hipHostFree(h->p);
+    for (DevBuf* d : {&s.d_tok, &s.d_neg, &s.d_len, &s.d_alpha, &s.d_off, &s.flags, &s.pos, &s.kept,
+                      &s.kept_off, &s.temp})
+      if (d->p) hipFree(d->p);
+    if (s.queue) hipFree(s.queue);
+    if (s.st) hipStreamDestroy(s.st);
+  }
+  for (void* q : {(void*)c->syn0, (void*)c->syn1, (void*)c->dsyn0, (void*)c->dsyn1, (void*)c->d_exp,
+                  (void*)c->d_nexp, (void*)c->d_counters, (void*)c->d_unigram, (void*)c->d_subs,
+                  (void*)c->d_corpus, (void*)c->d_off})
+    if (q) hipFree(q);
+  delete c;
+  return W2V_OK;
+}
+
+const char* w2v_last_error(const w2v_ctx* c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int w2v_sigmoid_tables(float* exp_table, float* nexp_table) {
+  if (!exp_table || !nexp_table) return W2V_ERR_ARG;
+  make_tables(exp_table, nexp_table);
+  return W2V_OK;
+}
+
+int w2v_sync_all(w2v_ctx* c) {
+  if (!c) return W2V_ERR_ARG;
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  for (auto& s : c->ss) {
+    HIP_TRY(c, hipStreamSynchronize(s.st));
+    int rc = drain_timing(c, s);
+    if (rc) return rc;
+  }
+  return W2V_OK;
+}
+
+int w2v_sync(w2v_ctx* c, int32_t stream) {
+  int rc = check_stream(c, stream);
+  if (rc) return rc;
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  HIP_TRY(c, hipStreamSynchronize(c->ss[stream].st));
+  return drain_timing(c, c->ss[stream]);
+}
+
+int w2v_model_upload(w2v_ctx* c, const float* syn0, const float* syn1neg) {
+  if (!c) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  const size_t d = c->p.layer_size, V = c->p.vocab_size, pitch = c->P * sizeof(float);
+  if (syn0) HIP_TRY(c, hipMemcpy2D(c->syn0, pitch, syn0, d * sizeof(float), d * sizeof(float), V, hipMemcpyHostToDevice));
+  if (syn1neg) HIP_TRY(c, hipMemcpy2D(c->syn1, pitch, syn1neg, d * sizeof(float), d * sizeof(float), V, hipMemcpyHostToDevice));
+  return W2V_OK;
+}
+
+int w2v_model_download(w2v_ctx* c, float* syn0, float* syn1neg) {
+  if (!c) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  const size_t d = c->p.layer_size, V = c->p.vocab_size, pitch = c->P * sizeof(float);
+  if (syn0) HIP_TRY(c, hipMemcpy2D(syn0, d * sizeof(float), c->syn0, pitch, d * sizeof(float), V, hipMemcpyDeviceToHost));
+  if (syn1neg) HIP_TRY(c, hipMemcpy2D(syn1neg, d * sizeof(float), c->syn1, pitch, d * sizeof(float), V, hipMemcpyDeviceToHost));
+  return W2V_OK;
+}
+
+int w2v_set_delta_mode(w2v_ctx* c, int32_t on) {
+  if (!c) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  const size_t rows = (size_t)c->p.vocab_size * c->P * sizeof(float);
+  if (on) {
+    if (!c->dsyn0) {
+      if (hipMalloc(&c->dsyn0, rows) != hipSuccess || hipMalloc(&c->dsyn1, rows) != hipSuccess)
+        return fail(c, W2V_ERR_NOMEM, "delta buffer allocation failed");
+    }
+    HIP_TRY(c, hipMemset(c->dsyn0, 0, rows));
+    HIP_TRY(c, hipMemset(c->dsyn1, 0, rows));
+  }
+  c->delta_mode = on ? 1 : 0;
+  return W2V_OK;
+}
+
+int w2v_delta_download(w2v_ctx* c, float* dsyn0, float* dsyn1neg) {
+  if (!c) return W2V_ERR_ARG;
+  if (!c->dsyn0) return fail(c, W2V_ERR_STATE, "delta mode was never enabled");
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  const size_t d = c->p.layer_size, V = c->p.vocab_size, pitch = c->P * sizeof(float);
+  if (dsyn0) HIP_TRY(c, hipMemcpy2D(dsyn0, d * sizeof(float), c->dsyn0, pitch, d * sizeof(float), V, hipMemcpyDeviceToHost));
+  if (dsyn1neg) HIP_TRY(c, hipMemcpy2D(dsyn1neg, d * sizeof(float), c->dsyn1, pitch, d * sizeof(float), V, hipMemcpyDeviceToHost));
+  return W2V_OK;
+}
+
+static int validate_ids(w2v_ctx* c, const int32_t* ids, int64_t n, const char* what) {
+  for (int64_t i = 0; i < n; i++)
+    if (ids[i] < 0 || ids[i] >= c->p.vocab_size)
+      return fail(c, W2V_ERR_ARG, "%s[%lld] = %d outside [0,%lld)", what, (long long)i, ids[i],
+                  (long long)c->p.vocab_size);
+  return W2V_OK;
+}
+
+int w2v_submit_csr(w2v_ctx* c, int32_t stream, const int32_t* tokens, const int64_t* offsets,
+                   int64_t n_sent, const int32_t* neg, const float* alpha) {
+  int rc = check_stream(c, stream);
+  if (rc) return rc;
+  if (n_sent < 0 || !offsets || !alpha || (n_sent > 0 && !tokens))
+    return fail(c, W2V_ERR_ARG, "invalid batch");
+  if (n_sent == 0) return W2V_OK;
+  const int K = c->p.negative;
+  if (K > 0 && !neg) return fail(c, W2V_ERR_ARG, "negatives required when negative > 0");
+  if (offsets[0] != 0) return fail(c, W2V_ERR_ARG, "offsets[0] must be 0");
+  for (int64_t s = 0; s < n_sent; s++)
+    if (offsets[s + 1] < offsets[s]) return fail(c, W2V_ERR_ARG, "offsets must be non-decreasing");
+  const int64_t nt = offsets[n_sent];
+  if ((rc = validate_ids(c, tokens, nt, "tokens"))) return rc;
+  if (K > 0 && (rc = validate_ids(c, neg, nt * K, "neg"))) return rc;
+  StreamState& s = c->ss[stream];
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  HIP_TRY(c, hipStreamSynchronize(s.st));  // previous batch on this stream released staging
+  const size_t bt = (size_t)nt * 4, bn = (size_t)nt * K * 4, bo = (size_t)(n_sent + 1) * 8,
+               ba = (size_t)n_sent * 4;
+  if ((rc = ensure_host(c, s.h_tok, bt)) || (rc = ensure_host(c, s.h_neg, bn)) ||
+      (rc = ensure_host(c, s.h_off, bo)) || (rc = ensure_host(c, s.h_alpha, ba)) ||
+      (rc = ensure_dev(c, s.d_tok, bt)) || (rc = ensure_dev(c, s.d_neg, bn)) ||
+      (rc = ensure_dev(c, s.d_off, bo)) || (rc = ensure_dev(c, s.d_alpha, ba)))
+    return rc;
+  memcpy(s.h_tok.p, tokens, bt);
+  if (bn) memcpy(s.h_neg.p, neg, bn);
+  memcpy(s.h_off.p, offsets, bo);
+  memcpy(s.h_alpha.p, alpha, ba);
+  if (bt) HIP_TRY(c, hipMemcpyAsync(s.d_tok.p, s.h_tok.p, bt, hipMemcpyHostToDevice, s.st));
+  if (bn) HIP_TRY(c, hipMemcpyAsync(s.d_neg.p, s.h_neg.p, bn, hipMemcpyHostToDevice, s.st));
+  HIP_TRY(c, hipMemcpyAsync(s.d_off.p, s.h_off.p, bo, hipMemcpyHostToDevice, s.st));
+  HIP_TRY(c, hipMemcpyAsync(s.d_alpha.p, s.h_alpha.p, ba, hipMemcpyHostToDevice, s.st));
+  UpdateArgs a = base_args(c);
+  a.tokens = (const int32_t*)s.d_tok.p;
+  a.offsets = (const int64_t*)s.d_off.p;
+  a.neg = K > 0 ? (const int32_t*)s.d_neg.p : nullptr;
+  a.alpha = (const float*)s.d_alpha.p;
+  a.n_sent = n_sent;
+  return launch_update(c, s, a);
+}
+
+int w2v_submit_padded(w2v_ctx* c, int32_t stream, const int32_t* sen, const int32_t* neg,
+                      const int32_t* length, const float* alpha, int32_t batch) {
+  int rc = check_stream(c, stream);
+  if (rc) return rc;
+  if (batch < 0 || !length || !alpha || (batch > 0 && !sen)) return fail(c, W2V_ERR_ARG, "invalid batch");
+  if (batch == 0) return W2V_OK;
+  const int K = c->p.negative, msl = c->p.max_sentence_length;
+  if (K > 0 && !neg) return fail(c, W2V_ERR_ARG, "negatives required when negative > 0");
+  for (int32_t b = 0; b < batch; b++) {
+    if (length[b] < 0 || length[b] > msl)
+      return fail(c, W2V_ERR_ARG, "length[%d] = %d outside [0,%d]", b, length[b], msl);
+    if ((rc = validate_ids(c, sen + (size_t)b * msl, length[b], "sen"))) return rc;
+    if (K > 0 && (rc = validate_ids(c, neg + (size_t)b * msl * K, (int64_t)length[b] * K, "neg"))) return rc;
+  }
+  StreamState& s = c->ss[stream];
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  HIP_TRY(c, hipStreamSynchronize(s.st));
+  const size_t bt = (size_t)batch * msl * 4, bn = bt * K, bl = (size_t)batch * 4, ba = bl;
+  if ((rc = ensure_host(c, s.h_tok, bt)) || (rc = ensure_host(c, s.h_neg, bn)) ||
+      (rc = ensure_host(c, s.h_len, bl)) || (rc = ensure_host(c, s.h_alpha, ba)) ||
+      (rc = ensure_dev(c, s.d_tok, bt)) || (rc = ensure_dev(c, s.d_neg, bn)) ||
+      (rc = ensure_dev(c, s.d_len, bl)) || (rc = ensure_dev(c, s.d_alpha, ba)))
+    return rc;
+  memcpy(s.h_tok.p, sen, bt);
+  if (bn) memcpy(s.h_neg.p, neg, bn);
+  memcpy(s.h_len.p, length, bl);
+  memcpy(s.h_alpha.p, alpha, ba);
+  HIP_TRY(c, hipMemcpyAsync(s.d_tok.p, s.h_tok.p, bt, hipMemcpyHostToDevice, s.st));
+  if (bn) HIP_TRY(c, hipMemcpyAsync(s.d_neg.p, s.h_neg.p, bn, hipMemcpyHostToDevice, s.st));
+  HIP_TRY(c, hipMemcpyAsync(s.d_len.p, s.h_len.p, bl, hipMemcpyHostToDevice, s.st));
+  HIP_TRY(c, hipMemcpyAsync(s.d_alpha.p, s.h_alpha.p, ba, hipMemcpyHostToDevice, s.st));
+  UpdateArgs a = base_args(c);
+  a.tokens = (const int32_t*)s.d_tok.p;
+  a.length = (const int32_t*)s.d_len.p;
+  a.neg = K > 0 ? (const int32_t*)s.d_neg.p : nullptr;
+  a.alpha = (const float*)s.d_alpha.p;
+  a.n_sent = batch;
+  return launch_update(c, s, a);
+}
+
+int w2v_tables_upload(w2v_ctx* c, const int32_t* unigram, int64_t n_unigram, const float* subsample_p) {
+  if (!c || (n_unigram > 0 && !unigram) || n_unigram < 0) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  if ((rc = validate_ids(c, unigram, n_unigram, "unigram"))) return rc;
+  if (c->d_unigram) HIP_TRY(c, hipFree(c->d_unigram));
+  c->d_unigram = nullptr;
+  c->n_unigram = n_unigram;
+  if (n_unigram > 0) {
+    if (hipMalloc(&c->d_unigram, (size_t)n_unigram * 4) != hipSuccess)
+      return fail(c, W2V_ERR_NOMEM, "unigram table allocation failed");
+    HIP_TRY(c, hipMemcpy(c->d_unigram, unigram, (size_t)n_unigram * 4, hipMemcpyHostToDevice));
+  }
+  if (c->d_subs) HIP_TRY(c, hipFree(c->d_subs));
+  c->d_subs = nullptr;
+  if (subsample_p) {
+    const size_t b = (size_t)c->p.vocab_size * 4;
+    if (hipMalloc(&c->d_subs, b) != hipSuccess) return fail(c, W2V_ERR_NOMEM, "subsample table allocation failed");
+    HIP_TRY(c, hipMemcpy(c->d_subs, subsample_p, b, hipMemcpyHostToDevice));
+  }
+  return W2V_OK;
+}
+
+int w2v_set_schedule(w2v_ctx* c, float alpha0, int32_t epochs, uint64_t train_words) {
+  if (!c || epochs < 1 || !(alpha0 > 0)) return W2V_ERR_ARG;
+  c->alpha0 = alpha0;
+  c->epochs = epochs;
+  c->train_words = train_words;
+  return W2V_OK;
+}
+
+int w2v_corpus_upload(w2v_ctx* c, const int32_t* tokens, int64_t n_tokens, const int64_t* off,
+                      int64_t n_sent) {
+  if (!c || n_tokens < 0 || n_sent < 0 || !off || (n_tokens > 0 && !tokens)) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  if (off[0] != 0 || off[n_sent] > n_tokens) return fail(c, W2V_ERR_ARG, "sentence offsets out of range");
+  for (int64_t s = 0; s < n_sent; s++)
+    if (off[s + 1] < off[s]) return fail(c, W2V_ERR_ARG, "offsets must be non-decreasing");
+  if ((rc = validate_ids(c, tokens, n_tokens, "tokens"))) return rc;
+  if (c->d_corpus) HIP_TRY(c, hipFree(c->d_corpus));
+  if (c->d_off) HIP_TRY(c, hipFree(c->d_off));
+  c->d_corpus = nullptr;
+  c->d_off = nullptr;
+  if (hipMalloc(&c->d_corpus, (size_t)(n_tokens > 0 ? n_tokens : 1) * 4) != hipSuccess ||
+      hipMalloc(&c->d_off, (size_t)(n_sent + 1) * 8) != hipSuccess)
+    return fail(c, W2V_ERR_NOMEM, "corpus allocation failed");
+  if (n_tokens) HIP_TRY(c, hipMemcpy(c->d_corpus, tokens, (size_t)n_tokens * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_off, off, (size_t)(n_sent + 1) * 8, hipMemcpyHostToDevice));
+  c->h_off.assign(off, off + n_sent + 1);
+  c->n_tokens = n_tokens;
+  c->n_sent = n_sent;
+  return W2V_OK;
+}
+
+int w2v_train_device(w2v_ctx* c, int32_t stream, int32_t epoch, int64_t sb, int64_t se) {
+  int rc = check_stream(c, stream);
+  if (rc) return rc;
+  if (!c->d_corpus) return fail(c, W2V_ERR_STATE, "no device corpus uploaded");
+  if (c->p.negative > 0 && !c->d_unigram) return fail(c, W2V_ERR_STATE, "no unigram table uploaded");
+  if (sb < 0 || se > c->n_sent || sb > se) return fail(c, W2V_ERR_ARG, "sentence range out of bounds");
+  if (epoch < 0 || epoch >= c->epochs) return fail(c, W2V_ERR_ARG, "epoch outside the schedule");
+  if (sb == se) return W2V_OK;
+  const int64_t tb = c->h_off[sb], n = c->h_off[se] - tb;
+  if (n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "at most 2^31-2 tokens per call");
+  StreamState& s = c->ss[stream];
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  const size_t tb_bytes = compact_temp_bytes(n);
+  if ((rc = ensure_dev(c, s.flags, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.pos, (size_t)(n + 1) * 4)) ||
+      (rc = ensure_dev(c, s.kept, (size_t)(n + 1) * 4)) ||
+      (rc = ensure_dev(c, s.kept_off, (size_t)(se - sb + 1) * 8)) || (rc = ensure_dev(c, s.temp, tb_bytes)))
+    return rc;
+  UpdateArgs a = base_args(c);
+  HIP_TRY(c, launch_compact(c->d_corpus, c->d_off, sb, se, c->d_subs, a.mixed_seed, epoch,
+                            (int32_t*)s.flags.p, (int32_t*)s.pos.p, (int32_t*)s.kept.p,
+                            (int64_t*)s.kept_off.p, s.temp.p, s.temp.cap, tb, n, s.st));
+  a.tokens = (const int32_t*)s.kept.p;
+  a.offsets = (const int64_t*)s.kept_off.p;
+  a.neg = nullptr;
+  a.alpha = nullptr;
+  a.unigram = c->d_unigram;
+  a.n_unigram = c->n_unigram;
+  a.epoch = epoch;
+  a.raw_offsets = c->d_off + sb;
+  a.alpha0 = c->alpha0;
+  a.total_words = (uint64_t)c->epochs * c->train_words + 1;
+  a.epoch_words = (uint64_t)epoch * c->train_words;
+  a.n_sent = se - sb;
+  c->words_read += (uint64_t)n;
+  return launch_update(c, s, a);
+}
+
+int w2v_get_stats(w2v_ctx* c, w2v_stats* out) {
+  if (!c || !out) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  unsigned long long h[4] = {0, 0, 0, 0};
+  HIP_TRY(c, hipMemcpy(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+  out->words_read = c->words_read;
+  out->words_trained = h[0];
+  out->sentences = h[1];
+  out->launches = c->launches;
+  return W2V_OK;
+}
+
+int w2v_kernel_time(w2v_ctx* c, int32_t stream, double* ms, uint64_t* launches) {
+  int rc = check_stream(c, stream);
+  if (rc) return rc;
+  if ((rc = w2v_sync(c, stream))) return rc;
+  if (ms) *ms = c->ss[stream].ms;
+  if (launches) *launches = c->ss[stream].launches;
+  return W2V_OK;
+}
+
+int w2v_heldout_loss(w2v_ctx* c, const int32_t* tokens, const int64_t* off, int64_t n_sent,
+                     double* loss, int64_t* pairs) {
+  if (!c || !off || n_sent < 0 || !loss) return W2V_ERR_ARG;
+  if (c->p.negative > 0 && !c->d_unigram) return fail(c, W2V_ERR_STATE, "no unigram table uploaded");
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  const int64_t nt = off[n_sent];
+  if ((rc = validate_ids(c, tokens, nt, "tokens"))) return rc;
+  int32_t* dt = nullptr;
+  int64_t* doff = nullptr;
+  double* acc = nullptr;
+  unsigned long long* np = nullptr;
+  if (hipMalloc(&dt, (size_t)(nt > 0 ? nt : 1) * 4) != hipSuccess ||
+      hipMalloc(&doff, (size_t)(n_sent + 1) * 8) != hipSuccess || hipMalloc(&acc, 8) != hipSuccess ||
+      hipMalloc(&np, 8) != hipSuccess) {
+    (void)hipFree(dt); (void)hipFree(doff); (void)hipFree(acc); (void)hipFree(np);
+    return fail(c, W2V_ERR_NOMEM, "held-out buffers allocation failed");
+  }
+  hipError_t e = hipSuccess;
+  if (nt) e = hipMemcpy(dt, tokens, (size_t)nt * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(doff, off, (size_t)(n_sent + 1) * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(acc, 0, 8);
+  if (e == hipSuccess) e = hipMemset(np, 0, 8);
+  if (e == hipSuccess)
+    e = launch_heldout_loss(c->syn0, c->syn1, c->P, c->p.window, c->p.negative, dt, doff, n_sent,
+                            c->d_unigram, c->n_unigram, mix64(c->p.seed), acc, np, c->num_cu, c->ss[0].st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->ss[0].st);
+  double hacc = 0.0;
+  unsigned long long hnp = 0;
+  if (e == hipSuccess) e = hipMemcpy(&hacc, acc, 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&hnp, np, 8, hipMemcpyDeviceToHost);
+  (void)hipFree(dt); (void)hipFree(doff); (void)hipFree(acc); (void)hipFree(np);
+  if (e != hipSuccess) return fail(c, W2V_ERR_HIP, "held-out loss failed: %s", hipGetErrorString(e));
+  *loss = hnp ? hacc / (double)hnp : 0.0;
+  if (pairs) *pairs = (int64_t)hnp;
+  return W2V_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// Auxiliary device kernels: work-queue reset, device subsampling and compaction of a
+// corpus range, and the held-out SGNS loss with an FP64 accumulator.
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+
+namespace w2v {
+
+__global__ void zero_u32_kernel(unsigned int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+// keep flag of corpus position tok_begin+i: the reference's subsampling test
+// (driver.cu:216-219) with the draw taken from the keyed stream instead of the
+// per-thread LCG. flags[n] = 0 so an exclusive scan of n+1 entries ends in the total.
+__global__ void keep_flags_kernel(const int32_t* __restrict__ corpus, int64_t tok_begin, int64_t n,
+                                  const float* __restrict__ subs_p, uint64_t mixed_seed,
+                                  int32_t epoch, int32_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    flags[i] = 0;
+    return;
+  }
+  int keep = 1;
+  if (subs_p) {
+    const int64_t pos = tok_begin + i;
+    const int32_t w = corpus[pos];
+    const uint64_t x = key2(mixed_seed, pos_key(epoch, pos), 0);
+    keep = !(subs_p[w] < (float)(x & 0xFFFF) / 65536.f);
+  }
+  flags[i] = keep;
+}
+
+__global__ void scatter_kept_kernel(const int32_t* __restrict__ corpus, int64_t tok_begin, int64_t n,
+                                    const int32_t* __restrict__ flags,
+                                    const int32_t* __restrict__ pos, int32_t* __restrict__ kept) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i]) kept[pos[i]] = corpus[tok_begin + i];
+}
+
+// kept_off[s] = number of kept tokens before sentence sb+s within the range
+__global__ void kept_offsets_kernel(const int64_t* __restrict__ raw_off, int64_t sb, int64_t ns,
+                                    int64_t tok_begin, const int32_t* __restrict__ pos,
+                                    int64_t* __restrict__ kept_off) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  kept_off[s] = (int64_t)pos[raw_off[sb + s] - tok_begin];
+}
+
+size_t compact_temp_bytes(int64_t n) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr,
+                                   (int)(n + 1));
+  return bytes;
+}
+
+hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t sb, int64_t se,
+                          const float* subs_p, uint64_t mixed_seed, int32_t epoch,
+                          int32_t* flags, int32_t* pos, int32_t* kept, int64_t* kept_off,
+                          void* temp, size_t temp_bytes, int64_t tok_begin, int64_t n,
+                          hipStream_t st) {
+  const int B = 256;
+  hipLaunchKernelGGL(keep_flags_kernel, dim3((unsigned)((n + 1 + B - 1) / B)), dim3(B), 0, st,
+                     corpus, tok_begin, n, subs_p, mixed_seed, epoch, flags);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, pos, (int)(n + 1), st);
+  if (err != hipSuccess) return err;
+  if (n > 0)
+    hipLaunchKernelGGL(scatter_kept_kernel, dim3((unsigned)((n + B - 1) / B)), dim3(B), 0, st,
+                       corpus, tok_begin, n, flags, pos, kept);
+  const int64_t ns = se - sb;
+  hipLaunchKernelGGL(kept_offsets_kernel, dim3((unsigned)((ns + 1 + B - 1) / B)), dim3(B), 0, st,
+                     raw_off, sb, ns, tok_begin, pos, kept_off);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Held-out loss: one wave per sentence, dots in FP32 lanes, per-pair loss and the
+// accumulation in FP64.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double neg_log_sigmoid(double x) {
+  return x > 0.0 ? log1p(exp(-x)) : -x + log1p(exp(x));
+}
+
+__global__ void __launch_bounds__(256) heldout_loss_kernel(
+    const float* __restrict__ syn0, const float* __restrict__ syn1, int P, int W, int K,
+    const int32_t* __restrict__ tokens, const int64_t* __restrict__ off, int64_t n_sent,
+    const int32_t* __restrict__ unigram, int64_t n_unigram, uint64_t mixed_seed,
+    double* __restrict__ acc, unsigned long long* __restrict__ pairs) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  double sum = 0.0;
+  unsigned long long np = 0;
+  for (int64_t s = wave; s < n_sent; s += nwaves) {
+    const int64_t b = off[s];
+    const int L = (int)(off[s + 1] - b);
+    for (int w = 0; w < L; w++) {
+      const int cw = tokens[b + w];
+      int negv = 0;
+      if (lane < K) {
+        const uint64_t x = key2(mixed_seed, pos_key(-1, b + w), (uint64_t)lane + 1);
+        negv = unigram[((x >> 32) * (uint64_t)n_unigram) >> 32];
+      }
+      for (int c = w - W; c <= w + W; c++) {
+        if (c < 0 || c >= L || c == w) continue;
+        const float* u = syn0 + (size_t)tokens[b + c] * P;
+        double l = 0.0;
+        for (int k = 0; k <= K; k++) {
+          const int t = k == 0 ? cw : readlane_i(negv, k - 1);
+          const float* v = syn1 + (size_t)t * P;
+          float part = 0.0f;
+          for (int j = lane; j < P; j += 64) part = fmaf(u[j], v[j], part);
+          const float f = wave_sum(part);
+          l += k == 0 ? neg_log_sigmoid((double)f) : neg_log_sigmoid(-(double)f);
+        }
+        sum += l;
+        np += 1;
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(acc, sum);
+    atomicAdd(pairs, np);
+  }
+}
+
+hipError_t launch_heldout_loss(const float* syn0, const float* syn1, int P, int W, int K,
+                               const int32_t* tokens, const int64_t* off, int64_t n_sent,
+                               const int32_t* unigram, int64_t n_unigram, uint64_t mixed_seed,
+                               double* acc, unsigned long long* pairs, int num_cu, hipStream_t st) {
+  int64_t blocks = (n_sent + 3) / 4;
+  if (blocks > (int64_t)num_cu * 8) blocks = (int64_t)num_cu * 8;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(heldout_loss_kernel, dim3((unsigned)blocks), dim3(256), 0, st, syn0, syn1, P,
+                     W, K, tokens, off, n_sent, unigram, n_unigram, mixed_seed, acc, pairs);
+  return hipGetLastError();
+}
+
+}  // namespace w2v

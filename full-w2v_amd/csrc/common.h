@@ -1,0 +1,119 @@
+// Shared device helpers and kernel argument blocks for libw2v (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace w2v {
+
+constexpr int kWave = 64;
+constexpr int kExpTableSize = 1000;
+
+// ---------------------------------------------------------------------------------
+// Wave-wide float sum, result in every lane. Row (16-lane) butterfly on DPP, then the
+// gfx950 permlane swaps across rows: no LDS and no barrier (the reference's
+// blockReduceM2, FULL-W2V-kernels.cu:74-90, needs a shared array and __syncthreads).
+// ---------------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror: every lane holds its 16-lane row sum
+  {
+    int a = __float_as_int(v);
+    auto r = __builtin_amdgcn_permlane16_swap(a, a, false, false);
+    v = __int_as_float(r[0]) + __int_as_float(r[1]);  // rows {0,1} and {2,3} summed
+  }
+  {
+    int a = __float_as_int(v);
+    auto r = __builtin_amdgcn_permlane32_swap(a, a, false, false);
+    v = __int_as_float(r[0]) + __int_as_float(r[1]);  // all four rows
+  }
+  return v;
+}
+
+__device__ __forceinline__ int readlane_i(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// ---------------------------------------------------------------------------------
+// Sigmoid-gradient. Table mode restates FULL-W2V-kernels.cu:92-99 (index rule) and
+// :153-164 (positive uses nexpTable with +alpha, negatives expTable with -alpha).
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int sigmoid_index(float f) {
+  float x = (f + 6.0f) * 0.083333336f;
+  x = (x >= 0.0f) ? x : 0.0f;  // saturate, NaN -> 0
+  x = (x > 1.0f) ? 1.0f : x;
+  return (int)(x * 999.0f);
+}
+
+// splitmix64 finaliser and the keyed stream used for device sampling.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// key(seed,a,b) = mix(mix(mix(seed) ^ a) ^ b); callers pass mixed_seed = mix(seed).
+__host__ __device__ __forceinline__ uint64_t key2(uint64_t mixed_seed, uint64_t a, uint64_t b) {
+  return mix64(mix64(mixed_seed ^ a) ^ b);
+}
+__host__ __device__ __forceinline__ uint64_t pos_key(int32_t epoch, int64_t pos) {
+  return ((uint64_t)(uint32_t)epoch << 40) ^ (uint64_t)pos;
+}
+
+// ---------------------------------------------------------------------------------
+// Kernel argument block of the sentence-update kernels.
+// ---------------------------------------------------------------------------------
+struct UpdateArgs {
+  // model (device layout: row r at r*P, pad columns d..P-1 kept at zero)
+  float* syn0;
+  float* syn1;
+  // flush targets: the model in normal mode, FP32 delta buffers in delta mode
+  float* out0;
+  float* out1;
+  // delta buffers added to model reads in delta mode (nullptr otherwise)
+  const float* rd0;
+  const float* rd1;
+  // sentences: CSR (offsets != nullptr) or padded (base = s*msl, len = length[s])
+  const int32_t* tokens;
+  const int64_t* offsets;
+  const int32_t* length;
+  const int32_t* neg;    // explicit negatives [(base+w)*K + k]; nullptr = device draws
+  const float* alpha;    // explicit per-sentence learning rate; nullptr = schedule
+  int64_t n_sent;
+  int32_t msl;
+  // device sampling (neg == nullptr)
+  const int32_t* unigram;
+  int64_t n_unigram;
+  uint64_t mixed_seed;
+  int32_t epoch;
+  const int64_t* raw_offsets;  // corpus position of each sentence's first token
+  float alpha0;
+  uint64_t total_words;  // epochs * train_words + 1 (schedule denominator)
+  uint64_t epoch_words;  // epoch * train_words (schedule offset)
+  // shape
+  int32_t d, P, W, K;
+  int32_t sig_exact;
+  int32_t fence;  // deterministic ordering of flushes before later reads
+  const float* exp_t;
+  const float* nexp_t;
+  // work queue and counters
+  unsigned int* queue;
+  unsigned long long* counters;  // [0] words trained, [1] sentences
+};
+
+}  // namespace w2v

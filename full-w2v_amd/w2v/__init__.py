@@ -1,0 +1,214 @@
+"""Python binding of libw2v (include/w2v.h) for tests and the benchmark.
+
+The trainer itself is the HIP library; this module only marshals numpy arrays through
+the C ABI with ctypes. Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libw2v.so")
+
+W2V_OK = 0
+ERRORS = {-1: "W2V_ERR_ARG", -2: "W2V_ERR_HIP", -3: "W2V_ERR_STATE", -4: "W2V_ERR_NOMEM",
+          -5: "W2V_ERR_UNSUPPORTED"}
+STORAGE = {"fp32": 0, "bf16": 1}
+KERNEL = {"plain": 0, "mfma": 1}
+SIGMOID = {"table": 0, "exact": 1}
+MODE = {"hogwild": 0, "deterministic": 1}
+
+# Every function declared in include/w2v.h, in declaration order.
+EXPORTS = [
+    "w2v_create", "w2v_destroy", "w2v_last_error", "w2v_model_upload", "w2v_model_download",
+    "w2v_sigmoid_tables", "w2v_submit_padded", "w2v_submit_csr", "w2v_sync", "w2v_sync_all",
+    "w2v_set_delta_mode", "w2v_delta_download", "w2v_tables_upload", "w2v_set_schedule",
+    "w2v_corpus_upload", "w2v_train_device", "w2v_get_stats", "w2v_heldout_loss",
+    "w2v_kernel_time",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("vocab_size", ctypes.c_int64), ("layer_size", ctypes.c_int32),
+        ("window", ctypes.c_int32), ("negative", ctypes.c_int32),
+        ("max_sentence_length", ctypes.c_int32), ("storage", ctypes.c_int32),
+        ("kernel", ctypes.c_int32), ("sigmoid", ctypes.c_int32), ("mode", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("streams", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("words_read", ctypes.c_uint64), ("words_trained", ctypes.c_uint64),
+                ("sentences", ctypes.c_uint64), ("launches", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libw2v.so from the package directory (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libw2v.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+        sig = {
+            "w2v_create": [ctypes.POINTER(Params), ctypes.POINTER(vp)],
+            "w2v_destroy": [vp],
+            "w2v_model_upload": [vp, vp, vp],
+            "w2v_model_download": [vp, vp, vp],
+            "w2v_sigmoid_tables": [vp, vp],
+            "w2v_submit_padded": [vp, i32, vp, vp, vp, vp, i32],
+            "w2v_submit_csr": [vp, i32, vp, vp, i64, vp, vp],
+            "w2v_sync": [vp, i32],
+            "w2v_sync_all": [vp],
+            "w2v_set_delta_mode": [vp, i32],
+            "w2v_delta_download": [vp, vp, vp],
+            "w2v_tables_upload": [vp, vp, i64, vp],
+            "w2v_set_schedule": [vp, ctypes.c_float, i32, u64],
+            "w2v_corpus_upload": [vp, vp, i64, vp, i64],
+            "w2v_train_device": [vp, i32, i32, i64, i64],
+            "w2v_get_stats": [vp, ctypes.POINTER(Stats)],
+            "w2v_heldout_loss": [vp, vp, vp, i64, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_int64)],
+            "w2v_kernel_time": [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.w2v_last_error.argtypes = [vp]
+        L.w2v_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+class W2VError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def sigmoid_tables():
+    e = np.zeros(1000, np.float32)
+    n = np.zeros(1000, np.float32)
+    rc = lib().w2v_sigmoid_tables(_ptr(e), _ptr(n))
+    if rc:
+        raise W2VError(rc, "w2v_sigmoid_tables")
+    return e, n
+
+
+class Trainer:
+    """One libw2v context (model on one GPU)."""
+
+    def __init__(self, vocab_size, layer_size, window=5, negative=5, msl=1000, storage="fp32",
+                 kernel="plain", sigmoid="table", mode="hogwild", device=0, streams=1, seed=1):
+        self.V, self.d, self.W, self.K, self.msl = int(vocab_size), int(layer_size), int(window), int(negative), int(msl)
+        p = Params(self.V, self.d, self.W, self.K, self.msl, STORAGE[storage], KERNEL[kernel],
+                   SIGMOID[sigmoid], MODE[mode], device, streams, 0, seed)
+        h = ctypes.c_void_p()
+        rc = lib().w2v_create(ctypes.byref(p), ctypes.byref(h))
+        if rc:
+            raise W2VError(rc, lib().w2v_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc:
+            raise W2VError(rc, lib().w2v_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().w2v_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, syn0=None, syn1=None):
+        s0 = None if syn0 is None else np.ascontiguousarray(syn0, np.float32)
+        s1 = None if syn1 is None else np.ascontiguousarray(syn1, np.float32)
+        self._chk(lib().w2v_model_upload(self.h, _ptr(s0), _ptr(s1)))
+
+    def download(self):
+        s0 = np.zeros((self.V, self.d), np.float32)
+        s1 = np.zeros((self.V, self.d), np.float32)
+        self._chk(lib().w2v_model_download(self.h, _ptr(s0), _ptr(s1)))
+        return s0, s1
+
+    def submit_csr(self, tokens, offsets, neg, alpha, stream=0):
+        t = np.ascontiguousarray(tokens, np.int32)
+        o = np.ascontiguousarray(offsets, np.int64)
+        n = None if neg is None else np.ascontiguousarray(neg, np.int32)
+        a = np.ascontiguousarray(alpha, np.float32)
+        self._chk(lib().w2v_submit_csr(self.h, stream, _ptr(t), _ptr(o), len(o) - 1, _ptr(n), _ptr(a)))
+
+    def submit_padded(self, sen, neg, length, alpha, stream=0):
+        s = np.ascontiguousarray(sen, np.int32)
+        n = None if neg is None else np.ascontiguousarray(neg, np.int32)
+        ln = np.ascontiguousarray(length, np.int32)
+        a = np.ascontiguousarray(alpha, np.float32)
+        self._chk(lib().w2v_submit_padded(self.h, stream, _ptr(s), _ptr(n), _ptr(ln), _ptr(a), len(ln)))
+
+    def sync(self, stream=None):
+        if stream is None:
+            self._chk(lib().w2v_sync_all(self.h))
+        else:
+            self._chk(lib().w2v_sync(self.h, stream))
+
+    def set_delta_mode(self, on=True):
+        self._chk(lib().w2v_set_delta_mode(self.h, 1 if on else 0))
+
+    def delta_download(self):
+        s0 = np.zeros((self.V, self.d), np.float32)
+        s1 = np.zeros((self.V, self.d), np.float32)
+        self._chk(lib().w2v_delta_download(self.h, _ptr(s0), _ptr(s1)))
+        return s0, s1
+
+    def tables_upload(self, unigram, subsample_p=None):
+        u = np.ascontiguousarray(unigram, np.int32)
+        sp = None if subsample_p is None else np.ascontiguousarray(subsample_p, np.float32)
+        self._chk(lib().w2v_tables_upload(self.h, _ptr(u), len(u), _ptr(sp)))
+
+    def set_schedule(self, alpha0, epochs, train_words):
+        self._chk(lib().w2v_set_schedule(self.h, float(alpha0), int(epochs), int(train_words)))
+
+    def corpus_upload(self, tokens, offsets):
+        t = np.ascontiguousarray(tokens, np.int32)
+        o = np.ascontiguousarray(offsets, np.int64)
+        self._chk(lib().w2v_corpus_upload(self.h, _ptr(t), len(t), _ptr(o), len(o) - 1))
+
+    def train_device(self, epoch, sent_begin, sent_end, stream=0):
+        self._chk(lib().w2v_train_device(self.h, stream, epoch, sent_begin, sent_end))
+
+    def stats(self):
+        s = Stats()
+        self._chk(lib().w2v_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def heldout_loss(self, tokens, offsets):
+        t = np.ascontiguousarray(tokens, np.int32)
+        o = np.ascontiguousarray(offsets, np.int64)
+        loss = ctypes.c_double()
+        pairs = ctypes.c_int64()
+        self._chk(lib().w2v_heldout_loss(self.h, _ptr(t), _ptr(o), len(o) - 1, ctypes.byref(loss),
+                                         ctypes.byref(pairs)))
+        return loss.value, pairs.value
+
+    def kernel_time(self, stream=0):
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        self._chk(lib().w2v_kernel_time(self.h, stream, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

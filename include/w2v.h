@@ -1,0 +1,150 @@
+/*
+ * libw2v — C ABI of the MI355X (gfx950) SGNS trainer.
+ *
+ * The reference binds its kernel at compile time: the driver #includes a kernels file
+ * that exports kernel_setup(), the function pointer cuda_kernel and device globals
+ * (src/FULL-W2V/FULL-W2V-kernels.cu:1-32), and the driver owns model allocation, upload
+ * and download (src/FULL-W2V/improved_driver_functions.cpp:11-33,228). This header is
+ * the run-time replacement for that boundary: plain pointers and sizes, status codes
+ * instead of exit(), no framework types.
+ *
+ * Conventions
+ *   - every function returns W2V_OK (0) or a negative W2V_ERR_* code; the message of
+ *     the last failure on a context is available from w2v_last_error();
+ *   - matrices cross the boundary as host row-major V x d float32 arrays; the library
+ *     keeps its own padded device layout;
+ *   - calls on distinct stream indices may come from distinct host threads; calls on
+ *     one stream index must be serialised by the caller;
+ *   - host arrays passed to submit functions are copied before the call returns.
+ */
+#ifndef W2V_H
+#define W2V_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define W2V_OK 0
+#define W2V_ERR_ARG (-1)         /* invalid argument */
+#define W2V_ERR_HIP (-2)         /* HIP runtime failure (message has the HIP error) */
+#define W2V_ERR_STATE (-3)       /* call not valid in the context's current state */
+#define W2V_ERR_NOMEM (-4)       /* host or device allocation failed */
+#define W2V_ERR_UNSUPPORTED (-5) /* shape outside the kernels' envelope */
+
+/* storage precision of syn0/syn1neg in device memory */
+#define W2V_STORAGE_FP32 0
+#define W2V_STORAGE_BF16 1
+/* update kernel */
+#define W2V_KERNEL_PLAIN 0 /* per-pair sequential update, FP32 VALU (the reference method) */
+#define W2V_KERNEL_MFMA 1  /* per-window tile product on bf16 MFMA, FP32 accumulation */
+/* sigmoid */
+#define W2V_SIGMOID_TABLE 0 /* the reference's 1000-entry table and index rule */
+#define W2V_SIGMOID_EXACT 1 /* 1/(1+exp(-f)) */
+/* scheduling */
+#define W2V_MODE_HOGWILD 0       /* many sentences in flight, unsynchronised atomics */
+#define W2V_MODE_DETERMINISTIC 1 /* one sentence at a time, in submission order */
+
+typedef struct w2v_ctx w2v_ctx;
+
+typedef struct w2v_params {
+  int64_t vocab_size;          /* V: rows of syn0 and syn1neg */
+  int32_t layer_size;          /* d: 1..1024 (reference: kernels.cu:20-30) */
+  int32_t window;              /* W >= 1 */
+  int32_t negative;            /* K >= 0 */
+  int32_t max_sentence_length; /* MSL of padded batches (driver.cu:144-147) */
+  int32_t storage;             /* W2V_STORAGE_* */
+  int32_t kernel;              /* W2V_KERNEL_* */
+  int32_t sigmoid;             /* W2V_SIGMOID_* */
+  int32_t mode;                /* W2V_MODE_* */
+  int32_t device;              /* HIP device ordinal */
+  int32_t streams;             /* submission streams, >= 1 (reference: -streams) */
+  int32_t reserved;
+  uint64_t seed;               /* device sampling and stochastic-rounding seed */
+} w2v_params;
+
+typedef struct w2v_stats {
+  uint64_t words_read;    /* corpus tokens read by device-sampled training (incl. dropped) */
+  uint64_t words_trained; /* kept centre tokens trained, all paths */
+  uint64_t sentences;     /* sentences trained, all paths */
+  uint64_t launches;      /* update-kernel launches */
+} w2v_stats;
+
+/* Replaces kernel_setup() (FULL-W2V-kernels.cu:15-32) and the device allocation of
+ * InitNet() (improved_driver_functions.cpp:11-12). Model memory is zeroed. */
+int w2v_create(const w2v_params* params, w2v_ctx** out);
+/* Replaces cudaFree(c_syn0/c_syn1neg) (FULL-W2V-driver.cu:469-470). NULL is a no-op. */
+int w2v_destroy(w2v_ctx* ctx);
+/* Message of the last failure on ctx (or of the last failed w2v_create when ctx is NULL). */
+const char* w2v_last_error(const w2v_ctx* ctx);
+
+/* Replaces the cudaMemcpy2DAsync uploads of InitNet (improved_driver_functions.cpp:25,32).
+ * Either pointer may be NULL to leave that matrix unchanged. */
+int w2v_model_upload(w2v_ctx* ctx, const float* syn0, const float* syn1neg);
+/* Replaces the cudaMemcpy2D download of saveModel (improved_driver_functions.cpp:228).
+ * Synchronises all streams first. Either pointer may be NULL. */
+int w2v_model_download(w2v_ctx* ctx, float* syn0, float* syn1neg);
+
+/* The sigmoid tables the kernels use: expTable/nexpTable of the reference
+ * (expTable.h, generated by generate_expTable_h.cpp:24-74 with size 1000). */
+int w2v_sigmoid_tables(float* exp_table, float* nexp_table);
+
+/* Replaces one batch launch of Train_SGNS (FULL-W2V-driver.cu:286-293): the four
+ * host-to-device copies and cuda_kernel<<<kernel_batch_size, d, smem, stream>>>(syn0,
+ * syn1neg, sen, neg, length, alpha, ...). Same padded layout as the reference:
+ * sen[batch][MSL], neg[batch][MSL][K], length[batch], alpha[batch]. */
+int w2v_submit_padded(w2v_ctx* ctx, int32_t stream, const int32_t* sen, const int32_t* neg,
+                      const int32_t* length, const float* alpha, int32_t batch);
+/* Same update on an unpadded batch: sentence s holds tokens[offsets[s]..offsets[s+1]),
+ * its negatives neg[(offsets[s]+w)*K + k], learning rate alpha[s]. */
+int w2v_submit_csr(w2v_ctx* ctx, int32_t stream, const int32_t* tokens, const int64_t* offsets,
+                   int64_t n_sentences, const int32_t* neg, const float* alpha);
+/* Replaces cudaStreamSynchronize on one submission stream (driver.cu:286). */
+int w2v_sync(w2v_ctx* ctx, int32_t stream);
+/* Replaces the final cudaDeviceSynchronize (driver.cu:323). */
+int w2v_sync_all(w2v_ctx* ctx);
+
+/* Delta mode (deterministic checks): when on, context and target deltas are
+ * accumulated into separate FP32 buffers (zeroed on entry) instead of the model, and
+ * model reads add those buffers, so deltas are exact sums of per-update terms. */
+int w2v_set_delta_mode(w2v_ctx* ctx, int32_t on);
+int w2v_delta_download(w2v_ctx* ctx, float* dsyn0, float* dsyn1neg);
+
+/* ---- device-resident corpus with device-side sampling ------------------------- */
+/* Unigram table (InitNet, improved_driver_functions.cpp:45-58) and subsampling keep
+ * probabilities (:71-77), produced by the host. subsample_p may be NULL (no sampling). */
+int w2v_tables_upload(w2v_ctx* ctx, const int32_t* unigram, int64_t n_unigram,
+                      const float* subsample_p);
+/* Learning-rate schedule of Train_SGNS (driver.cu:188-189): alpha0, total epochs and
+ * the corpus word count used in the decay. */
+int w2v_set_schedule(w2v_ctx* ctx, float alpha0, int32_t epochs, uint64_t train_words);
+/* Corpus sentences (the reference's global_sentences over global_stream,
+ * improved_driver_functions.cpp:244-281) copied into device memory. */
+int w2v_corpus_upload(w2v_ctx* ctx, const int32_t* tokens, int64_t n_tokens,
+                      const int64_t* sent_offsets, int64_t n_sentences);
+/* Trains sentences [sent_begin, sent_end) of the device corpus for the given epoch:
+ * subsampling and negatives are drawn on the device from counter-based streams, the
+ * learning rate follows the schedule at each sentence's corpus position. Asynchronous
+ * on the given stream. Replaces the host batching loop for a device-resident corpus. */
+int w2v_train_device(w2v_ctx* ctx, int32_t stream, int32_t epoch, int64_t sent_begin,
+                     int64_t sent_end);
+
+/* Counters (read after synchronising all streams). */
+int w2v_get_stats(w2v_ctx* ctx, w2v_stats* out);
+
+/* Held-out SGNS loss of the current model with an FP64 accumulator on the device:
+ * mean over (context, centre) pairs of -log s(u_c.v_w) - sum_k log s(-u_c.v_nk). No
+ * subsampling; negatives from the device stream with epoch -1. */
+int w2v_heldout_loss(w2v_ctx* ctx, const int32_t* tokens, const int64_t* offsets,
+                     int64_t n_sentences, double* loss, int64_t* pairs);
+
+/* Device time of the update kernels launched so far on a stream (milliseconds, HIP
+ * events around each launch) and their number. */
+int w2v_kernel_time(w2v_ctx* ctx, int32_t stream, double* ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* W2V_H */

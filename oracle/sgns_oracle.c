@@ -1,0 +1,742 @@
+/*
+ * ORACLE — test infrastructure only (see sgns_oracle.h).
+ * CPU restatement of the reference SGNS method; every function cites the reference
+ * file:line it follows. Never linked into the product library.
+ */
+#define _GNU_SOURCE
+#include "sgns_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <sched.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define LCG_A 25214903917ULL
+#define LCG_C 11ULL
+
+/* ------------------------------------------------------------------------------------ */
+/* sigmoid tables — generate_expTable_h.cpp:24-74 (float arithmetic, MAX_EXP = 6)        */
+/* ------------------------------------------------------------------------------------ */
+static float gen_val(int i, int size) {
+  /* generate_expTable_h.cpp:48-49: val = exp((i/(real)size*2-1)*MAX_EXP); val/(val+1) */
+  float val = expf((i / (float)size * 2 - 1) * 6);
+  val = val / (val + 1);
+  return val;
+}
+
+void orc_exp_tables(float* exp_table, float* nexp_table, int size) {
+  /* expTable: index i holds sigma(x_i); ends forced to 0 and 1 (:50-51) */
+  for (int i = 0; i < size; i++) {
+    if (i == 0) exp_table[i] = 0.0f;
+    else if (i + 1 == size) exp_table[i] = 1.0f;
+    else exp_table[i] = gen_val(i, size);
+  }
+  /* nexpTable is emitted for i = size-1 down to 0 (:60-71): position j = size-1-i */
+  for (int i = size - 1; i >= 0; i--) {
+    int j = size - 1 - i;
+    if (i == size - 1) nexp_table[j] = 1.0f;
+    else if (i == 0) nexp_table[j] = 0.0f;
+    else nexp_table[j] = gen_val(i, size);
+  }
+}
+
+int orc_sigmoid_index(float f) {
+  /* FULL-W2V-kernels.cu:92-99 with dilute = 1/(2*MAX_EXP) = 0.083333336f and
+   * expand = 2*MAX_EXP*EXP_RESOLUTION = 999.0f (driver.cu:18,55) */
+  float g = (f + 6.0f) * 0.083333336f;
+  if (!(g >= 0.0f)) g = 0.0f; /* __saturatef: NaN -> 0 */
+  else if (g > 1.0f) g = 1.0f;
+  g = g * 999.0f;
+  return (int)g;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* LCG — FULL-W2V-driver.cu:217,225; seed = thread id (:114)                             */
+/* ------------------------------------------------------------------------------------ */
+uint64_t orc_lcg_next(uint64_t x) { return x * LCG_A + LCG_C; }
+
+void orc_lcg_stream(uint64_t seed, int64_t n, uint64_t* out) {
+  uint64_t x = seed;
+  for (int64_t i = 0; i < n; i++) {
+    x = orc_lcg_next(x);
+    out[i] = x;
+  }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* unigram table — improved_driver_functions.cpp:45-58                                   */
+/* ------------------------------------------------------------------------------------ */
+void orc_unigram_table(const uint64_t* cn, int32_t V, int32_t* table, int64_t n, int corrected) {
+  double pow_sum = 0.0;
+  for (int32_t i = 0; i < V; i++) pow_sum += pow((double)cn[i], 0.75);
+  if (!corrected) {
+    /* :50-51 — threshold starts at p_0 while the first label is 1 (label shift) */
+    int32_t i = 0;
+    double d1 = pow((double)cn[i], 0.75) / pow_sum;
+    i = 1;
+    for (int64_t a = 0; a < n; a++) {
+      table[a] = i;
+      if (a / (double)n > d1) {
+        if (i < V - 1) i++;
+        d1 += pow((double)cn[i], 0.75) / pow_sum;
+      }
+    }
+  } else {
+    /* corrected: label i receives p_i (word2vec's original table) */
+    int32_t i = 0;
+    double d1 = pow((double)cn[0], 0.75) / pow_sum;
+    for (int64_t a = 0; a < n; a++) {
+      table[a] = i;
+      if (a / (double)n > d1) {
+        if (i < V - 1) i++;
+        d1 += pow((double)cn[i], 0.75) / pow_sum;
+      }
+    }
+  }
+}
+
+/* improved_driver_functions.cpp:71-77 (float arithmetic) */
+void orc_subsample_probs(const uint64_t* cn, int32_t V, float sample, uint64_t train_words,
+                         float* p) {
+  float base = sample * (float)train_words;
+  for (int32_t i = 0; i < V; i++) {
+    float ratio = base / (float)cn[i];
+    p[i] = sqrtf(ratio) + ratio;
+  }
+}
+
+/* improved_driver_functions.cpp:27-31 */
+void orc_init_syn0(float* syn0, int64_t V, int32_t d) {
+  uint64_t x = 1;
+  int64_t n = V * (int64_t)d;
+  for (int64_t i = 0; i < n; i++) {
+    x = orc_lcg_next(x);
+    syn0[i] = (((x & 0xFFFF) / 65536.f) - 0.5f) / d;
+  }
+}
+
+/* FULL-W2V-driver.cu:188-189 */
+float orc_learning_rate(float alpha0, uint64_t wca, int32_t epochs, uint64_t train_words) {
+  float a = alpha0 * (1 - wca / (float)((uint64_t)epochs * train_words + 1));
+  if (a < alpha0 * 0.0001f) a = alpha0 * 0.0001f;
+  return a;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* FP64 serialised SGNS of one sentence — FULL-W2V-kernels.cu:106-250                    */
+/* Windows: textbook [w-W, w+W] ∩ [0,L) \ {w}; contexts visited left to right.           */
+/* ------------------------------------------------------------------------------------ */
+static float g_exp[1000], g_nexp[1000];
+static pthread_once_t g_tables_once = PTHREAD_ONCE_INIT;
+static void init_tables(void) { orc_exp_tables(g_exp, g_nexp, 1000); }
+
+static double sig_grad(double f, int positive, double alpha, int sig_mode) {
+  if (sig_mode == 0) {
+    /* kernels.cu:94-98 and :153-164: positive uses nexpTable with +alpha, negatives
+     * use expTable with -alpha. The index is computed from the FP32 dot. */
+    int idx = orc_sigmoid_index((float)f);
+    return positive ? alpha * (double)g_nexp[idx] : -alpha * (double)g_exp[idx];
+  }
+  /* exact: positive alpha*(1-s(f)) = alpha*s(-f); negative -alpha*s(f) */
+  return positive ? alpha / (1.0 + exp(f)) : -alpha / (1.0 + exp(-f));
+}
+
+static double bf16_round(double v) {
+  float f = (float)v;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return v; /* inf / nan untouched */
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
+                           const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
+                           int sig_mode, int tile, int bf16_emul) {
+  pthread_once(&g_tables_once, init_tables);
+  if (L <= 0) return;
+  /* per-position working copies U and load-time snapshots U0 (lifetime reuse) */
+  double* U = (double*)calloc((size_t)L * d, sizeof(double));
+  double* U0 = (double*)calloc((size_t)L * d, sizeof(double));
+  double* T = (double*)calloc((size_t)(K + 1) * d, sizeof(double));
+  double* T0 = (double*)calloc((size_t)(K + 1) * d, sizeof(double));
+  double* G = (double*)calloc((size_t)(2 * W + 1) * (K + 1), sizeof(double));
+  double* dU = (double*)calloc((size_t)(2 * W + 1) * d, sizeof(double));
+  int32_t* tgt = (int32_t*)calloc((size_t)K + 1, sizeof(int32_t));
+
+  /* preload positions 0..W (kernels.cu:139-145) */
+  for (int32_t p = 0; p <= W && p < L; p++)
+    for (int32_t j = 0; j < d; j++) U[(size_t)p * d + j] = U0[(size_t)p * d + j] = syn0[(size_t)sen[p] * d + j];
+
+  for (int32_t w = 0; w < L; w++) {
+    int32_t lo = w - W < 0 ? 0 : w - W;
+    int32_t hi = w + W > L - 1 ? L - 1 : w + W;
+    for (int32_t k = 0; k <= K; k++) tgt[k] = k == 0 ? sen[w] : neg[(size_t)w * K + (k - 1)];
+    if (!tile) {
+      /* kernels.cu:151-215: each target trained against every context in turn */
+      for (int32_t k = 0; k <= K; k++) {
+        double* t = T;
+        for (int32_t j = 0; j < d; j++) t[j] = T0[j] = syn1[(size_t)tgt[k] * d + j];
+        for (int32_t c = lo; c <= hi; c++) {
+          if (c == w) continue;
+          double* u = U + (size_t)c * d;
+          double f = 0.0;
+          for (int32_t j = 0; j < d; j++) f += u[j] * t[j];
+          double g = sig_grad(f, k == 0, alpha, sig_mode);
+          for (int32_t j = 0; j < d; j++) {
+            double uj = u[j], tj = t[j];
+            u[j] = uj + g * tj; /* simultaneous update, kernels.cu:174-175 */
+            t[j] = tj + g * uj;
+          }
+        }
+        for (int32_t j = 0; j < d; j++) syn1[(size_t)tgt[k] * d + j] += t[j] - T0[j];
+      }
+    } else {
+      /* window tile: S = C T^T from one snapshot, G = g(S), dC = G T, dT = G^T C */
+      for (int32_t k = 0; k <= K; k++)
+        for (int32_t j = 0; j < d; j++) T[(size_t)k * d + j] = syn1[(size_t)tgt[k] * d + j];
+      int32_t nc = 0;
+      for (int32_t c = lo; c <= hi; c++) {
+        if (c == w) continue;
+        const double* u = U + (size_t)c * d;
+        for (int32_t k = 0; k <= K; k++) {
+          const double* t = T + (size_t)k * d;
+          double f = 0.0;
+          for (int32_t j = 0; j < d; j++)
+            f += (bf16_emul ? bf16_round(u[j]) * bf16_round(t[j]) : u[j] * t[j]);
+          double g = sig_grad(bf16_emul ? (double)(float)f : f, k == 0, alpha, sig_mode);
+          G[(size_t)nc * (K + 1) + k] = bf16_emul ? bf16_round(g) : g;
+        }
+        nc++;
+      }
+      /* dC */
+      nc = 0;
+      for (int32_t c = lo; c <= hi; c++) {
+        if (c == w) continue;
+        double* du = dU + (size_t)nc * d;
+        for (int32_t j = 0; j < d; j++) {
+          double acc = 0.0;
+          for (int32_t k = 0; k <= K; k++) {
+            double tj = T[(size_t)k * d + j];
+            acc += G[(size_t)nc * (K + 1) + k] * (bf16_emul ? bf16_round(tj) : tj);
+          }
+          du[j] = acc;
+        }
+        nc++;
+      }
+      /* dT, applied to the model per target in order (duplicates accumulate) */
+      for (int32_t k = 0; k <= K; k++) {
+        for (int32_t j = 0; j < d; j++) {
+          double acc = 0.0;
+          int32_t q = 0;
+          for (int32_t c = lo; c <= hi; c++) {
+            if (c == w) continue;
+            double uj = U[(size_t)c * d + j];
+            acc += G[(size_t)q * (K + 1) + k] * (bf16_emul ? bf16_round(uj) : uj);
+            q++;
+          }
+          syn1[(size_t)tgt[k] * d + j] += acc;
+        }
+      }
+      nc = 0;
+      for (int32_t c = lo; c <= hi; c++) {
+        if (c == w) continue;
+        for (int32_t j = 0; j < d; j++) U[(size_t)c * d + j] += dU[(size_t)nc * d + j];
+        nc++;
+      }
+    }
+    /* ring advance (kernels.cu:217-238): flush the leaving context, then load the
+     * entering one, so a duplicate entering word sees the flushed delta */
+    int32_t leave = w - W, enter = w + W + 1;
+    if (leave >= 0)
+      for (int32_t j = 0; j < d; j++)
+        syn0[(size_t)sen[leave] * d + j] += U[(size_t)leave * d + j] - U0[(size_t)leave * d + j];
+    if (enter < L)
+      for (int32_t j = 0; j < d; j++)
+        U[(size_t)enter * d + j] = U0[(size_t)enter * d + j] = syn0[(size_t)sen[enter] * d + j];
+  }
+  /* final flush of the positions still in the window (kernels.cu:245-249) */
+  for (int32_t p = (L - W > 0 ? L - W : 0); p < L; p++)
+    for (int32_t j = 0; j < d; j++) syn0[(size_t)sen[p] * d + j] += U[(size_t)p * d + j] - U0[(size_t)p * d + j];
+
+  free(U); free(U0); free(T); free(T0); free(G); free(dU); free(tgt);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Sentence assembly — FULL-W2V-driver.cu:166-302 (one host thread, one epoch)           */
+/* ------------------------------------------------------------------------------------ */
+typedef struct {
+  uint64_t* wca;       /* shared word_count_actual */
+  float* alpha;        /* shared alpha */
+  pthread_mutex_t* mu; /* NULL when single threaded */
+} shared_lr;
+
+/* Assembles the next GPU sentence into tok/ng. Returns its length, or -1 at epoch end
+ * (after which the caller records a zero-length sentence, driver.cu:233-283). */
+typedef struct {
+  const int32_t* stream;
+  const int64_t* sent_start;
+  int64_t slice_begin, slice_end;
+  int32_t msl, K;
+  float sample;
+  const float* subs_p;
+  const int32_t* unigram;
+  int64_t n_unigram;
+  float alpha0;
+  int32_t epochs;
+  uint64_t train_words;
+  /* per-thread loop state */
+  uint64_t next_random, word_count, last_word_count;
+  int64_t stream_sentence, current_position;
+} asm_state;
+
+static void lr_update(asm_state* a, shared_lr* s) {
+  /* driver.cu:168-190 */
+  if (a->word_count - a->last_word_count > 10000) {
+    uint64_t diff = a->word_count - a->last_word_count;
+    if (s->mu) pthread_mutex_lock(s->mu);
+    *s->wca += diff;
+    uint64_t wca = *s->wca;
+    if (s->mu) pthread_mutex_unlock(s->mu);
+    a->last_word_count = a->word_count;
+    float al = a->alpha0 * (1 - wca / (float)((uint64_t)a->epochs * a->train_words + 1));
+    if (al < a->alpha0 * 0.0001f) al = a->alpha0 * 0.0001f;
+    *s->alpha = al;
+  }
+}
+
+/* returns sentence length; *epoch_end set when the slice is exhausted */
+static int32_t assemble_one(asm_state* a, shared_lr* s, int32_t* tok, int32_t* ng, int* epoch_end) {
+  int32_t len = 0;
+  *epoch_end = 0;
+  /* an empty slice ends the epoch without reading (the reference would read past it) */
+  while (len < a->msl && a->stream_sentence < a->slice_end) {
+    if (a->current_position >= a->msl) {
+      a->stream_sentence++;
+      a->current_position = 0;
+      if (a->stream_sentence >= a->slice_end) break;
+    }
+    int64_t w = a->stream[a->sent_start[a->stream_sentence] + a->current_position];
+    a->current_position++;
+    a->word_count++;
+    if (w == 0 || a->stream_sentence >= a->slice_end) {
+      a->stream_sentence++;
+      a->current_position = 0;
+      break;
+    }
+    if (a->sample > 0) {
+      a->next_random = orc_lcg_next(a->next_random);
+      if (a->subs_p[w] < (a->next_random & 0xFFFF) / 65536.f) continue;
+    }
+    tok[len] = (int32_t)w;
+    for (int32_t k = 0; k < a->K; k++) {
+      a->next_random = orc_lcg_next(a->next_random);
+      ng[(size_t)len * a->K + k] = a->unigram[(a->next_random >> 16) % (uint64_t)a->n_unigram];
+    }
+    len++;
+  }
+  if (a->stream_sentence >= a->slice_end) {
+    /* driver.cu:233-248: epoch end; the partial sentence is dropped (length reset) */
+    uint64_t diff = a->word_count - a->last_word_count;
+    if (s->mu) pthread_mutex_lock(s->mu);
+    *s->wca += diff;
+    if (s->mu) pthread_mutex_unlock(s->mu);
+    a->word_count = 0;
+    a->last_word_count = 0;
+    a->stream_sentence = a->slice_begin;
+    a->current_position = 0;
+    *epoch_end = 1;
+    return 0;
+  }
+  return len;
+}
+
+int64_t orc_assemble_epoch(const int32_t* stream, const int64_t* sent_start, int64_t slice_begin,
+                           int64_t slice_end, int32_t msl, int32_t K, float sample,
+                           const float* subs_p, const int32_t* unigram, int64_t n_unigram,
+                           float alpha0, int32_t epochs, uint64_t train_words,
+                           orc_thread_state* st, int32_t* out_tokens, int32_t* out_neg,
+                           int64_t* out_off, float* out_alpha, int64_t cap_tokens,
+                           int64_t cap_sent) {
+  asm_state a = {stream, sent_start, slice_begin, slice_end, msl, K, sample, subs_p, unigram,
+                 n_unigram, alpha0, epochs, train_words, st->next_random, st->word_count,
+                 st->last_word_count, slice_begin, 0};
+  shared_lr s = {&st->word_count_actual, &st->alpha, NULL};
+  int64_t ns = 0, nt = 0;
+  out_off[0] = 0;
+  int32_t* tok = (int32_t*)malloc(sizeof(int32_t) * (size_t)msl);
+  int32_t* ng = (int32_t*)malloc(sizeof(int32_t) * (size_t)msl * (K > 0 ? K : 1));
+  for (;;) {
+    lr_update(&a, &s);
+    int ee;
+    int32_t len = assemble_one(&a, &s, tok, ng, &ee);
+    if (ns >= cap_sent || nt + len > cap_tokens) { ns = -1; break; }
+    memcpy(out_tokens + nt, tok, sizeof(int32_t) * (size_t)len);
+    memcpy(out_neg + nt * K, ng, sizeof(int32_t) * (size_t)len * K);
+    nt += len;
+    out_alpha[ns] = *s.alpha; /* driver.cu:282 */
+    out_off[++ns] = nt;
+    if (ee) break;
+  }
+  free(tok);
+  free(ng);
+  st->next_random = a.next_random;
+  st->word_count = a.word_count;
+  st->last_word_count = a.last_word_count;
+  return ns;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* FP32 Hogwild trainer (multithreaded, in place) — same method as the kernel           */
+/* ------------------------------------------------------------------------------------ */
+static inline float sig_grad_f(float f, int positive, float alpha, int sig_mode) {
+  if (sig_mode == 0) {
+    int idx = orc_sigmoid_index(f);
+    return positive ? alpha * g_nexp[idx] : -alpha * g_exp[idx];
+  }
+  return positive ? alpha / (1.0f + expf(f)) : -alpha / (1.0f + expf(-f));
+}
+
+static void sgns_sentence_f32(float* syn0, float* syn1, int32_t d, int32_t W, int32_t K,
+                              const int32_t* sen, int32_t L, const int32_t* neg, float alpha,
+                              int sig_mode, float* ring, float* ring0, float* t, float* t0) {
+  const int32_t R = 2 * W + 1;
+  if (L <= 0) return;
+  for (int32_t p = 0; p <= W && p < L; p++) {
+    memcpy(ring + (size_t)(p % R) * d, syn0 + (size_t)sen[p] * d, sizeof(float) * d);
+    memcpy(ring0 + (size_t)(p % R) * d, syn0 + (size_t)sen[p] * d, sizeof(float) * d);
+  }
+  for (int32_t w = 0; w < L; w++) {
+    int32_t lo = w - W < 0 ? 0 : w - W;
+    int32_t hi = w + W > L - 1 ? L - 1 : w + W;
+    for (int32_t k = 0; k <= K; k++) {
+      int32_t tg = k == 0 ? sen[w] : neg[(size_t)w * K + (k - 1)];
+      float* row = syn1 + (size_t)tg * d;
+      memcpy(t, row, sizeof(float) * d);
+      memcpy(t0, row, sizeof(float) * d);
+      for (int32_t c = lo; c <= hi; c++) {
+        if (c == w) continue;
+        float* u = ring + (size_t)(c % R) * d;
+        float f = 0.0f;
+        for (int32_t j = 0; j < d; j++) f += u[j] * t[j];
+        float g = sig_grad_f(f, k == 0, alpha, sig_mode);
+        for (int32_t j = 0; j < d; j++) {
+          float uj = u[j], tj = t[j];
+          u[j] = uj + g * tj;
+          t[j] = tj + g * uj;
+        }
+      }
+      for (int32_t j = 0; j < d; j++) row[j] += t[j] - t0[j];
+    }
+    int32_t leave = w - W, enter = w + W + 1;
+    if (leave >= 0) {
+      float* dst = syn0 + (size_t)sen[leave] * d;
+      const float* u = ring + (size_t)(leave % R) * d;
+      const float* u0 = ring0 + (size_t)(leave % R) * d;
+      for (int32_t j = 0; j < d; j++) dst[j] += u[j] - u0[j];
+    }
+    if (enter < L) {
+      memcpy(ring + (size_t)(enter % R) * d, syn0 + (size_t)sen[enter] * d, sizeof(float) * d);
+      memcpy(ring0 + (size_t)(enter % R) * d, syn0 + (size_t)sen[enter] * d, sizeof(float) * d);
+    }
+  }
+  for (int32_t p = (L - W > 0 ? L - W : 0); p < L; p++) {
+    float* dst = syn0 + (size_t)sen[p] * d;
+    const float* u = ring + (size_t)(p % R) * d;
+    const float* u0 = ring0 + (size_t)(p % R) * d;
+    for (int32_t j = 0; j < d; j++) dst[j] += u[j] - u0[j];
+  }
+}
+
+typedef struct {
+  int id;
+  const orc_train_cfg* cfg;
+  const int32_t* stream;
+  const int64_t* sent_start;
+  int64_t n_sent;
+  uint64_t train_words;
+  const int32_t* unigram;
+  int64_t n_unigram;
+  const float* subs_p;
+  float* syn0;
+  float* syn1;
+  int64_t max_sent;
+  shared_lr* lr;
+  pthread_barrier_t* bar;
+  struct timespec* t_start;
+  int ncpu;
+  int* cpus;
+} hog_arg;
+
+static void* hog_thread(void* p) {
+  hog_arg* h = (hog_arg*)p;
+  const orc_train_cfg* c = h->cfg;
+  if (h->ncpu > 0) { /* pin one thread per core (driver.cu:97-100) */
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    CPU_SET(h->cpus[h->id % h->ncpu], &set);
+    sched_setaffinity(0, sizeof(set), &set);
+  }
+  /* driver.cu:104-110 — contiguous sentence slice per thread */
+  int64_t per = h->n_sent / c->threads, left = h->n_sent % c->threads, beg = 0;
+  for (int64_t i = 0; i < h->id; i++) beg += per + (i < left ? 1 : 0);
+  int64_t end = beg + per + (h->id < left ? 1 : 0);
+  asm_state a = {h->stream, h->sent_start, beg, end, c->msl, c->K, c->sample, h->subs_p,
+                 h->unigram, h->n_unigram, c->alpha0, c->epochs, h->train_words,
+                 (uint64_t)h->id, 0, 0, beg, 0};
+  int32_t R = 2 * c->W + 1;
+  int32_t* tok = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->msl);
+  int32_t* ng = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->msl * (c->K > 0 ? c->K : 1));
+  float* ring = (float*)malloc(sizeof(float) * (size_t)R * c->d);
+  float* ring0 = (float*)malloc(sizeof(float) * (size_t)R * c->d);
+  float* t = (float*)malloc(sizeof(float) * (size_t)c->d);
+  float* t0 = (float*)malloc(sizeof(float) * (size_t)c->d);
+  pthread_barrier_wait(h->bar);
+  if (h->id == 0) clock_gettime(CLOCK_MONOTONIC_RAW, h->t_start);
+  int64_t done = 0;
+  for (int32_t e = 0; e < c->epochs; e++) {
+    for (;;) {
+      lr_update(&a, h->lr);
+      int ee;
+      int32_t len = assemble_one(&a, h->lr, tok, ng, &ee);
+      float alpha = *h->lr->alpha;
+      if (len > 0)
+        sgns_sentence_f32(h->syn0, h->syn1, c->d, c->W, c->K, tok, len, ng, alpha, c->sig_mode,
+                          ring, ring0, t, t0);
+      if (ee) break;
+      if (h->max_sent >= 0 && ++done >= h->max_sent) {
+        /* bounded sample: account the words read so far and stop */
+        uint64_t diff = a.word_count - a.last_word_count;
+        pthread_mutex_lock(h->lr->mu);
+        *h->lr->wca += diff;
+        pthread_mutex_unlock(h->lr->mu);
+        a.last_word_count = a.word_count;
+        e = c->epochs;
+        break;
+      }
+    }
+  }
+  free(tok); free(ng); free(ring); free(ring0); free(t); free(t0);
+  return NULL;
+}
+
+double orc_train_hogwild(const orc_train_cfg* cfg, const int32_t* stream,
+                         const int64_t* sent_start, int64_t n_sent, uint64_t train_words,
+                         const int32_t* unigram, int64_t n_unigram, const float* subs_p,
+                         float* syn0, float* syn1, int64_t max_sentences_per_thread,
+                         uint64_t* words_out, double* seconds_out) {
+  pthread_once(&g_tables_once, init_tables);
+  int nt = cfg->threads > 0 ? cfg->threads : 1;
+  uint64_t wca = 0;
+  float alpha = cfg->alpha0;
+  pthread_mutex_t mu;
+  pthread_mutex_init(&mu, NULL);
+  shared_lr lr = {&wca, &alpha, &mu};
+  pthread_barrier_t bar;
+  pthread_barrier_init(&bar, NULL, (unsigned)nt);
+  struct timespec t0, t1;
+  cpu_set_t allowed;
+  int cpus[1024], ncpu = 0;
+  if (sched_getaffinity(0, sizeof(allowed), &allowed) == 0)
+    for (int i = 0; i < 1024 && ncpu < 1024; i++)
+      if (CPU_ISSET(i, &allowed)) cpus[ncpu++] = i;
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nt);
+  hog_arg* args = (hog_arg*)malloc(sizeof(hog_arg) * nt);
+  for (int i = 0; i < nt; i++) {
+    hog_arg h = {i, cfg, stream, sent_start, n_sent, train_words, unigram, n_unigram, subs_p,
+                 syn0, syn1, max_sentences_per_thread, &lr, &bar, &t0, ncpu, cpus};
+    args[i] = h;
+    pthread_create(&th[i], NULL, hog_thread, &args[i]);
+  }
+  for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+  clock_gettime(CLOCK_MONOTONIC_RAW, &t1);
+  double secs = (t1.tv_sec - t0.tv_sec) + (t1.tv_nsec - t0.tv_nsec) / 1e9;
+  pthread_barrier_destroy(&bar);
+  pthread_mutex_destroy(&mu);
+  free(th);
+  free(args);
+  if (words_out) *words_out = wca;
+  if (seconds_out) *seconds_out = secs;
+  return secs > 0 ? wca / secs : 0.0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* counter-based RNG (splitmix64 finaliser) and device sampling draws                    */
+/* ------------------------------------------------------------------------------------ */
+uint64_t orc_mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+uint64_t orc_key(uint64_t seed, uint64_t a, uint64_t b) {
+  return orc_mix64(orc_mix64(orc_mix64(seed) ^ a) ^ b);
+}
+
+static uint64_t pos_key(int32_t epoch, int64_t pos) {
+  return ((uint64_t)(uint32_t)epoch << 40) ^ (uint64_t)pos;
+}
+
+int32_t orc_device_keep(uint64_t seed, int32_t epoch, int64_t pos, int32_t word, const float* subs_p) {
+  uint64_t x = orc_key(seed, pos_key(epoch, pos), 0);
+  return !(subs_p[word] < (x & 0xFFFF) / 65536.f);
+}
+
+int32_t orc_device_negative(uint64_t seed, int32_t epoch, int64_t pos, int32_t k,
+                            const int32_t* unigram, int64_t n_unigram) {
+  uint64_t x = orc_key(seed, pos_key(epoch, pos), (uint64_t)k + 1);
+  uint64_t idx = ((x >> 32) * (uint64_t)n_unigram) >> 32;
+  return unigram[idx];
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* synthetic cluster-Markov Zipf corpus                                                  */
+/* ------------------------------------------------------------------------------------ */
+#define T53 9007199254740992.0 /* 2^53 */
+#define SALT_CLUSTER 0x636C7573746572ULL
+#define SALT_SENT 0x53454E54ULL
+#define SALT_LEN 0x4C454EULL
+
+int32_t orc_synth_cluster_of(const orc_synth_cfg* c, int32_t word) {
+  return (int32_t)(orc_key(c->seed, SALT_CLUSTER, (uint64_t)word) % (uint64_t)c->clusters);
+}
+
+static uint64_t thr_of(double cum, double total) {
+  double q = floor(cum / total * T53);
+  if (q >= T53) return (uint64_t)T53;
+  return (uint64_t)q;
+}
+
+int orc_synth_tables(const orc_synth_cfg* c, uint64_t* gthr, int32_t* members, int64_t* cstart,
+                     uint64_t* cthr, uint64_t* lthr) {
+  int32_t V = c->V, C = c->clusters;
+  if (V < 2 || C < 1) return -1;
+  double total = 0.0;
+  for (int32_t r = 1; r < V; r++) total += pow((double)r, -c->zipf_s);
+  double cum = 0.0;
+  gthr[0] = 0;
+  for (int32_t r = 1; r < V; r++) {
+    cum += pow((double)r, -c->zipf_s);
+    gthr[r] = (r == V - 1) ? (uint64_t)T53 : thr_of(cum, total);
+  }
+  /* cluster membership (counting sort by cluster, rank order kept) */
+  int64_t* cnt = (int64_t*)calloc((size_t)C + 1, sizeof(int64_t));
+  for (int32_t r = 1; r < V; r++) cnt[orc_synth_cluster_of(c, r) + 1]++;
+  cstart[0] = 0;
+  for (int32_t k = 0; k < C; k++) cstart[k + 1] = cstart[k] + cnt[k + 1];
+  int64_t* fill = (int64_t*)calloc((size_t)C, sizeof(int64_t));
+  for (int32_t r = 1; r < V; r++) {
+    int32_t k = orc_synth_cluster_of(c, r);
+    members[cstart[k] + fill[k]++] = r;
+  }
+  for (int32_t k = 0; k < C; k++) {
+    double tot = 0.0;
+    for (int64_t i = cstart[k]; i < cstart[k + 1]; i++) tot += pow((double)members[i], -c->zipf_s);
+    double cu = 0.0;
+    for (int64_t i = cstart[k]; i < cstart[k + 1]; i++) {
+      cu += pow((double)members[i], -c->zipf_s);
+      cthr[i] = (i == cstart[k + 1] - 1) ? (uint64_t)T53 : thr_of(cu, tot);
+    }
+  }
+  free(cnt);
+  free(fill);
+  /* sentence lengths: round(lognormal(ln median, sigma)) clamped to [1, max] */
+  int32_t M = c->len_max;
+  lthr[0] = 0;
+  if (c->len_median <= 0) {
+    for (int32_t L = 1; L <= M; L++) lthr[L] = (L == M) ? (uint64_t)T53 : 0;
+  } else {
+    double mu = log((double)c->len_median), sg = c->len_sigma;
+    for (int32_t L = 1; L <= M; L++) {
+      double z = (log(L + 0.5) - mu) / sg;
+      double cdf = 0.5 * erfc(-z / sqrt(2.0));
+      lthr[L] = (L == M) ? (uint64_t)T53 : thr_of(cdf, 1.0);
+    }
+  }
+  return 0;
+}
+
+static int64_t lower_bound_u64(const uint64_t* thr, int64_t lo, int64_t hi, uint64_t u) {
+  /* smallest i in [lo, hi) with u < thr[i] */
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (u < thr[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+int32_t orc_synth_length(const orc_synth_cfg* c, const uint64_t* lthr, int64_t s) {
+  uint64_t u = orc_key(c->seed, SALT_LEN, (uint64_t)s) >> 11;
+  return (int32_t)lower_bound_u64(lthr, 1, (int64_t)c->len_max + 1, u);
+}
+
+void orc_synth_sentence(const orc_synth_cfg* c, const uint64_t* gthr, const int32_t* members,
+                        const int64_t* cstart, const uint64_t* cthr, int64_t s, int32_t len,
+                        int32_t* out) {
+  uint64_t ss = orc_key(c->seed, SALT_SENT, (uint64_t)s);
+  uint64_t stay_thr = (uint64_t)floor(c->stay * T53);
+  uint64_t in_thr = (uint64_t)floor(c->in_cluster * T53);
+  int32_t C = c->clusters;
+  int32_t cl = (int32_t)(orc_key(ss, 0, 2) % (uint64_t)C);
+  for (int32_t i = 0; i < len; i++) {
+    if (i > 0 && (orc_key(ss, (uint64_t)i, 1) >> 11) >= stay_thr)
+      cl = (int32_t)(orc_key(ss, (uint64_t)i, 2) % (uint64_t)C);
+    uint64_t u2 = orc_key(ss, (uint64_t)i, 3) >> 11;
+    uint64_t u3 = orc_key(ss, (uint64_t)i, 4) >> 11;
+    if (u2 < in_thr && cstart[cl + 1] > cstart[cl]) {
+      int64_t idx = lower_bound_u64(cthr, cstart[cl], cstart[cl + 1], u3);
+      out[i] = members[idx];
+    } else {
+      out[i] = (int32_t)lower_bound_u64(gthr, 1, c->V, u3);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* held-out SGNS loss (FP64)                                                              */
+/* ------------------------------------------------------------------------------------ */
+static double nls(double x) { /* -log(sigmoid(x)), stable */
+  return x > 0 ? log1p(exp(-x)) : -x + log1p(exp(x));
+}
+
+double orc_heldout_loss(const float* syn0, const float* syn1, int32_t d, int32_t W, int32_t K,
+                        const int32_t* tokens, const int64_t* off, int64_t n_sent,
+                        const int32_t* unigram, int64_t n_unigram, uint64_t seed,
+                        int64_t* pairs_out) {
+  double sum = 0.0;
+  int64_t pairs = 0;
+  int32_t* ng = (int32_t*)malloc(sizeof(int32_t) * (size_t)(K > 0 ? K : 1));
+  for (int64_t s = 0; s < n_sent; s++) {
+    int64_t b = off[s];
+    int32_t L = (int32_t)(off[s + 1] - b);
+    for (int32_t w = 0; w < L; w++) {
+      int32_t cw = tokens[b + w];
+      for (int32_t k = 0; k < K; k++) ng[k] = orc_device_negative(seed, -1, b + w, k, unigram, n_unigram);
+      for (int32_t c = w - W; c <= w + W; c++) {
+        if (c < 0 || c >= L || c == w) continue;
+        const float* u = syn0 + (size_t)tokens[b + c] * d;
+        const float* v = syn1 + (size_t)cw * d;
+        double f = 0.0;
+        for (int32_t j = 0; j < d; j++) f += (double)u[j] * v[j];
+        double l = nls(f);
+        for (int32_t k = 0; k < K; k++) {
+          const float* vn = syn1 + (size_t)ng[k] * d;
+          double fn = 0.0;
+          for (int32_t j = 0; j < d; j++) fn += (double)u[j] * vn[j];
+          l += nls(-fn);
+        }
+        sum += l;
+        pairs++;
+      }
+    }
+  }
+  free(ng);
+  if (pairs_out) *pairs_out = pairs;
+  return pairs ? sum / pairs : 0.0;
+}

@@ -1,0 +1,138 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A plain-C restatement of the reference SGNS method (tlranda/FULL-W2V) used as the
+ * checker for the HIP product path. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it. The product library never links or calls it.
+ *
+ * Each function cites the reference file:line it restates (paths relative to the
+ * reference repository root).
+ *
+ * Parity pins (what checks each function independently of this file):
+ *   orc_exp_tables       byte-exact against the compiled reference generator
+ *                        (oracle/_ref) and a committed sha256; closed form sigma(x).
+ *   orc_sigmoid_index    brute-force sweep against the closed-form bucket edges.
+ *   orc_lcg_*            jump-ahead closed form x_n = A^n x_0 + c (A^n-1)/(A-1) mod 2^64.
+ *   orc_unigram_table    per-label share equals p_i (label shift: label 1 gets p_0,
+ *                        last label absorbs p_1), brute-force small vocabularies.
+ *   orc_subsample_probs  closed form sqrt(r)+r.
+ *   orc_init_syn0        closed form over the LCG stream (seed 1).
+ *   orc_learning_rate    closed form with floor.
+ *   orc_sgns_sentence_f64  hand closed forms (one pair, one window), brute-force
+ *                        autograd-free finite checks, invariants (untouched rows).
+ *   orc_assemble_*       hand-traced small corpora (counts, sentence breaks).
+ *   orc_synth_*          statistical properties (Zipf slope, stay probability).
+ *   orc_heldout_loss     closed form on a 1-pair corpus.
+ */
+#ifndef SGNS_ORACLE_H
+#define SGNS_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- sigmoid tables: generate_expTable_h.cpp:24-74, kernels.cu:92-99 ---- */
+void orc_exp_tables(float* exp_table, float* nexp_table, int size);
+int orc_sigmoid_index(float f);
+
+/* ---- LCG: FULL-W2V-driver.cu:114,217,225 ---- */
+uint64_t orc_lcg_next(uint64_t x);
+void orc_lcg_stream(uint64_t seed, int64_t n, uint64_t* out);
+
+/* ---- tables: improved_driver_functions.cpp:27-31,45-58,71-77 ---- */
+void orc_unigram_table(const uint64_t* cn, int32_t V, int32_t* table, int64_t n, int corrected);
+void orc_subsample_probs(const uint64_t* cn, int32_t V, float sample, uint64_t train_words, float* p);
+void orc_init_syn0(float* syn0, int64_t V, int32_t d);
+float orc_learning_rate(float alpha0, uint64_t word_count_actual, int32_t epochs, uint64_t train_words);
+
+/* ---- FP64 serialised SGNS update of one sentence: FULL-W2V-kernels.cu:106-250 ----
+ * sig_mode: 0 = reference table sigmoid, 1 = exact sigmoid.
+ * tile:     0 = sequential pair updates (reference order),
+ *           1 = window tile (all (context,target) dots of one centre from one snapshot).
+ * bf16_emul (tile only): round the tile-product operands to bfloat16 (RNE) as the
+ *           MFMA kernel does. */
+void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
+                           const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
+                           int sig_mode, int tile, int bf16_emul);
+
+/* ---- sentence assembly: FULL-W2V-driver.cu:102-114,166-302 ----
+ * Restates one host thread's batching loop over a sentence slice for one epoch:
+ * subsampling, negative draws and the per-sentence learning rate. Outputs GPU
+ * sentences in CSR form. Returns the number of GPU sentences written, or -1 when an
+ * output capacity is exceeded. `state` carries {next_random, word_count,
+ * last_word_count, word_count_actual} across calls (one epoch per call). */
+typedef struct {
+  uint64_t next_random;
+  uint64_t word_count;
+  uint64_t last_word_count;
+  uint64_t word_count_actual;
+  float alpha;
+} orc_thread_state;
+
+int64_t orc_assemble_epoch(const int32_t* stream, const int64_t* sent_start, int64_t slice_begin,
+                           int64_t slice_end, int32_t msl, int32_t K, float sample,
+                           const float* subs_p, const int32_t* unigram, int64_t n_unigram,
+                           float alpha0, int32_t epochs, uint64_t train_words,
+                           orc_thread_state* st, int32_t* out_tokens, int32_t* out_neg,
+                           int64_t* out_off, float* out_alpha, int64_t cap_tokens,
+                           int64_t cap_sent);
+
+/* ---- FP32 multithreaded Hogwild trainer (the CPU reference trainer) ---- */
+typedef struct {
+  int32_t d, W, K, msl, epochs, threads, sig_mode;
+  float alpha0, sample;
+} orc_train_cfg;
+
+/* Trains in place on syn0/syn1 (V x d, row-major). Returns words/sec (reference
+ * definition: every corpus token read, over wall time of the training threads).
+ * max_sentences_per_thread < 0 means no limit (used to time a bounded sample). */
+double orc_train_hogwild(const orc_train_cfg* cfg, const int32_t* stream,
+                         const int64_t* sent_start, int64_t n_sent, uint64_t train_words,
+                         const int32_t* unigram, int64_t n_unigram, const float* subs_p,
+                         float* syn0, float* syn1, int64_t max_sentences_per_thread,
+                         uint64_t* words_out, double* seconds_out);
+
+/* ---- counter-based RNG used by the device sampling path and the synthetic corpora ---- */
+uint64_t orc_mix64(uint64_t x);
+uint64_t orc_key(uint64_t seed, uint64_t a, uint64_t b);
+int32_t orc_device_keep(uint64_t seed, int32_t epoch, int64_t pos, int32_t word, const float* subs_p);
+int32_t orc_device_negative(uint64_t seed, int32_t epoch, int64_t pos, int32_t k,
+                            const int32_t* unigram, int64_t n_unigram);
+
+/* ---- synthetic cluster-Markov Zipf corpus ---- */
+typedef struct {
+  int32_t V;          /* ids 1..V-1 are words; id 0 is </s> */
+  int32_t clusters;
+  double zipf_s;
+  double stay;        /* probability of staying in the current cluster */
+  double in_cluster;  /* probability of drawing from the cluster's Zipf */
+  int32_t len_median; /* lognormal median; 0 = fixed length len_max */
+  double len_sigma;
+  int32_t len_max;
+  uint64_t seed;
+} orc_synth_cfg;
+
+int32_t orc_synth_cluster_of(const orc_synth_cfg* c, int32_t word);
+/* Integer thresholds (53-bit) for the samplers. Returns 0 on success. Arrays:
+ *   gthr[V] global cumulative thresholds (index = word id, gthr[0] = 0),
+ *   members[V-1] word ids grouped by cluster, cstart[clusters+1], cthr[V-1],
+ *   lthr[len_max+1] sentence-length thresholds. */
+int orc_synth_tables(const orc_synth_cfg* c, uint64_t* gthr, int32_t* members, int64_t* cstart,
+                     uint64_t* cthr, uint64_t* lthr);
+int32_t orc_synth_length(const orc_synth_cfg* c, const uint64_t* lthr, int64_t s);
+void orc_synth_sentence(const orc_synth_cfg* c, const uint64_t* gthr, const int32_t* members,
+                        const int64_t* cstart, const uint64_t* cthr, int64_t s, int32_t len,
+                        int32_t* out);
+
+/* ---- held-out SGNS loss (FP64) ----
+ * Mean over (context, centre) pairs of  -log s(u_c.v_w) - sum_k log s(-u_c.v_nk),
+ * negatives from orc_device_negative(seed, epoch=-1, pos, k). */
+double orc_heldout_loss(const float* syn0, const float* syn1, int32_t d, int32_t W, int32_t K,
+                        const int32_t* tokens, const int64_t* off, int64_t n_sent,
+                        const int32_t* unigram, int64_t n_unigram, uint64_t seed,
+                        int64_t* pairs_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

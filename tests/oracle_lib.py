@@ -1,0 +1,226 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) — test infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+vp = ctypes.c_void_p
+
+
+def P(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+class ThreadState(ctypes.Structure):
+    _fields_ = [("next_random", ctypes.c_uint64), ("word_count", ctypes.c_uint64),
+                ("last_word_count", ctypes.c_uint64), ("word_count_actual", ctypes.c_uint64),
+                ("alpha", ctypes.c_float)]
+
+
+class TrainCfg(ctypes.Structure):
+    _fields_ = [("d", ctypes.c_int32), ("W", ctypes.c_int32), ("K", ctypes.c_int32),
+                ("msl", ctypes.c_int32), ("epochs", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("sig_mode", ctypes.c_int32), ("alpha0", ctypes.c_float), ("sample", ctypes.c_float)]
+
+
+class SynthCfg(ctypes.Structure):
+    _fields_ = [("V", ctypes.c_int32), ("clusters", ctypes.c_int32), ("zipf_s", ctypes.c_double),
+                ("stay", ctypes.c_double), ("in_cluster", ctypes.c_double),
+                ("len_median", ctypes.c_int32), ("len_sigma", ctypes.c_double),
+                ("len_max", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        L = ctypes.CDLL(ORACLE_SO)
+        i32, i64, u64, f32, f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
+        L.orc_exp_tables.argtypes = [vp, vp, ctypes.c_int]
+        L.orc_sigmoid_index.argtypes = [f32]
+        L.orc_sigmoid_index.restype = ctypes.c_int
+        L.orc_lcg_next.argtypes = [u64]
+        L.orc_lcg_next.restype = u64
+        L.orc_lcg_stream.argtypes = [u64, i64, vp]
+        L.orc_unigram_table.argtypes = [vp, i32, vp, i64, ctypes.c_int]
+        L.orc_subsample_probs.argtypes = [vp, i32, f32, u64, vp]
+        L.orc_init_syn0.argtypes = [vp, i64, i32]
+        L.orc_learning_rate.argtypes = [f32, u64, i32, u64]
+        L.orc_learning_rate.restype = f32
+        L.orc_sgns_sentence_f64.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, f64, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int]
+        L.orc_assemble_epoch.argtypes = [vp, vp, i64, i64, i32, i32, f32, vp, vp, i64, f32, i32, u64,
+                                         ctypes.POINTER(ThreadState), vp, vp, vp, vp, i64, i64]
+        L.orc_assemble_epoch.restype = i64
+        L.orc_train_hogwild.argtypes = [ctypes.POINTER(TrainCfg), vp, vp, i64, u64, vp, i64, vp, vp, vp,
+                                        i64, ctypes.POINTER(u64), ctypes.POINTER(f64)]
+        L.orc_train_hogwild.restype = f64
+        L.orc_mix64.argtypes = [u64]
+        L.orc_mix64.restype = u64
+        L.orc_key.argtypes = [u64, u64, u64]
+        L.orc_key.restype = u64
+        L.orc_device_keep.argtypes = [u64, i32, i64, i32, vp]
+        L.orc_device_keep.restype = i32
+        L.orc_device_negative.argtypes = [u64, i32, i64, i32, vp, i64]
+        L.orc_device_negative.restype = i32
+        L.orc_synth_cluster_of.argtypes = [ctypes.POINTER(SynthCfg), i32]
+        L.orc_synth_cluster_of.restype = i32
+        L.orc_synth_tables.argtypes = [ctypes.POINTER(SynthCfg), vp, vp, vp, vp, vp]
+        L.orc_synth_tables.restype = ctypes.c_int
+        L.orc_synth_length.argtypes = [ctypes.POINTER(SynthCfg), vp, i64]
+        L.orc_synth_length.restype = i32
+        L.orc_synth_sentence.argtypes = [ctypes.POINTER(SynthCfg), vp, vp, vp, vp, i64, i32, vp]
+        L.orc_heldout_loss.argtypes = [vp, vp, i32, i32, i32, vp, vp, i64, vp, i64, u64,
+                                       ctypes.POINTER(i64)]
+        L.orc_heldout_loss.restype = f64
+        _lib = L
+    return _lib
+
+
+# ---- numpy conveniences ---------------------------------------------------------------
+def exp_tables():
+    e = np.zeros(1000, np.float32)
+    n = np.zeros(1000, np.float32)
+    lib().orc_exp_tables(P(e), P(n), 1000)
+    return e, n
+
+
+def lcg_stream(seed, n):
+    out = np.zeros(n, np.uint64)
+    lib().orc_lcg_stream(seed, n, P(out))
+    return out
+
+
+def unigram_table(cn, n, corrected=False):
+    cn = np.ascontiguousarray(cn, np.uint64)
+    t = np.zeros(n, np.int32)
+    lib().orc_unigram_table(P(cn), len(cn), P(t), n, int(corrected))
+    return t
+
+
+def subsample_probs(cn, sample, train_words):
+    cn = np.ascontiguousarray(cn, np.uint64)
+    p = np.zeros(len(cn), np.float32)
+    lib().orc_subsample_probs(P(cn), len(cn), sample, train_words, P(p))
+    return p
+
+
+def init_syn0(V, d):
+    s = np.zeros((V, d), np.float32)
+    lib().orc_init_syn0(P(s), V, d)
+    return s
+
+
+def sgns_sentence(syn0, syn1, W, K, sen, neg, alpha, sig_mode=1, tile=0, bf16=0):
+    """FP64 update of one sentence in place on float64 copies; returns (syn0', syn1')."""
+    s0 = np.array(syn0, np.float64, order="C")
+    s1 = np.array(syn1, np.float64, order="C")
+    sen = np.ascontiguousarray(sen, np.int32)
+    neg = np.ascontiguousarray(neg if neg is not None else np.zeros(0), np.int32)
+    lib().orc_sgns_sentence_f64(P(s0), P(s1), s0.shape[1], W, K, P(sen), len(sen), P(neg), alpha,
+                                sig_mode, tile, bf16)
+    return s0, s1
+
+
+def assemble_epoch(stream, sent_start, slice_begin, slice_end, msl, K, sample, subs_p, unigram,
+                   alpha0, epochs, train_words, state=None):
+    stream = np.ascontiguousarray(stream, np.int32)
+    sent_start = np.ascontiguousarray(sent_start, np.int64)
+    subs_p = np.ascontiguousarray(subs_p if subs_p is not None else np.zeros(1), np.float32)
+    unigram = np.ascontiguousarray(unigram, np.int32)
+    st = state or ThreadState(0, 0, 0, 0, alpha0)
+    cap_t = len(stream) + 1
+    cap_s = len(stream) + 8
+    tok = np.zeros(cap_t, np.int32)
+    ng = np.zeros(cap_t * max(K, 1), np.int32)
+    off = np.zeros(cap_s + 1, np.int64)
+    al = np.zeros(cap_s, np.float32)
+    ns = lib().orc_assemble_epoch(P(stream), P(sent_start), slice_begin, slice_end, msl, K, sample,
+                                  P(subs_p), P(unigram), len(unigram), alpha0, epochs, train_words,
+                                  ctypes.byref(st), P(tok), P(ng), P(off), P(al), cap_t, cap_s)
+    assert ns >= 0
+    nt = off[ns]
+    return tok[:nt].copy(), ng[:nt * K].copy(), off[:ns + 1].copy(), al[:ns].copy(), st
+
+
+def train_hogwild(stream, sent_start, train_words, unigram, subs_p, syn0, syn1, d, W, K, msl,
+                  epochs, threads, alpha0=0.025, sample=1e-4, sig_mode=0, max_sent=-1):
+    cfg = TrainCfg(d, W, K, msl, epochs, threads, sig_mode, alpha0, sample)
+    stream = np.ascontiguousarray(stream, np.int32)
+    sent_start = np.ascontiguousarray(sent_start, np.int64)
+    unigram = np.ascontiguousarray(unigram, np.int32)
+    subs_p = np.ascontiguousarray(subs_p, np.float32)
+    words = ctypes.c_uint64()
+    secs = ctypes.c_double()
+    wps = lib().orc_train_hogwild(ctypes.byref(cfg), P(stream), P(sent_start), len(sent_start),
+                                  train_words, P(unigram), len(unigram), P(subs_p), P(syn0), P(syn1),
+                                  max_sent, ctypes.byref(words), ctypes.byref(secs))
+    return wps, words.value, secs.value
+
+
+def synth_cfg(V, clusters=64, zipf_s=1.0, stay=0.9, in_cluster=0.8, len_median=22, len_sigma=1.0,
+              len_max=1000, seed=1):
+    return SynthCfg(V, clusters, zipf_s, stay, in_cluster, len_median, len_sigma, len_max, seed)
+
+
+def synth_tables(cfg):
+    gthr = np.zeros(cfg.V, np.uint64)
+    members = np.zeros(cfg.V - 1, np.int32)
+    cstart = np.zeros(cfg.clusters + 1, np.int64)
+    cthr = np.zeros(cfg.V - 1, np.uint64)
+    lthr = np.zeros(cfg.len_max + 1, np.uint64)
+    rc = lib().orc_synth_tables(ctypes.byref(cfg), P(gthr), P(members), P(cstart), P(cthr), P(lthr))
+    assert rc == 0
+    return gthr, members, cstart, cthr, lthr
+
+
+def synth_corpus(cfg, n_sent):
+    """Returns (tokens, offsets) of sentences 0..n_sent-1."""
+    gthr, members, cstart, cthr, lthr = synth_tables(cfg)
+    lens = np.array([lib().orc_synth_length(ctypes.byref(cfg), P(lthr), s) for s in range(n_sent)],
+                    np.int64)
+    off = np.zeros(n_sent + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    tok = np.zeros(off[-1], np.int32)
+    for s in range(n_sent):
+        buf = np.zeros(int(lens[s]), np.int32)
+        lib().orc_synth_sentence(ctypes.byref(cfg), P(gthr), P(members), P(cstart), P(cthr), s,
+                                 int(lens[s]), P(buf))
+        tok[off[s]:off[s + 1]] = buf
+    return tok, off
+
+
+def heldout_loss(syn0, syn1, W, K, tokens, off, unigram, seed):
+    syn0 = np.ascontiguousarray(syn0, np.float32)
+    syn1 = np.ascontiguousarray(syn1, np.float32)
+    tokens = np.ascontiguousarray(tokens, np.int32)
+    off = np.ascontiguousarray(off, np.int64)
+    unigram = np.ascontiguousarray(unigram, np.int32)
+    pairs = ctypes.c_int64()
+    loss = lib().orc_heldout_loss(P(syn0), P(syn1), syn0.shape[1], W, K, P(tokens), P(off),
+                                  len(off) - 1, P(unigram), len(unigram), seed, ctypes.byref(pairs))
+    return loss, pairs.value
+
+
+def device_negatives(seed, epoch, pos0, L, K, unigram):
+    unigram = np.ascontiguousarray(unigram, np.int32)
+    out = np.zeros((L, K), np.int32)
+    for w in range(L):
+        for k in range(K):
+            out[w, k] = lib().orc_device_negative(seed, epoch, pos0 + w, k, P(unigram), len(unigram))
+    return out
+
+
+def device_keep(seed, epoch, tokens, pos0, subs_p):
+    subs_p = np.ascontiguousarray(subs_p, np.float32)
+    return np.array([lib().orc_device_keep(seed, epoch, pos0 + i, int(w), P(subs_p))
+                     for i, w in enumerate(tokens)], bool)

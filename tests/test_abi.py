@@ -1,0 +1,55 @@
+"""The C-ABI library loads and exports every function include/w2v.h declares (CPU)."""
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "w2v.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return re.findall(r"^\s*(?:int|const char\*)\s+(w2v_\w+)\s*\(", txt, flags=re.M)
+
+
+def test_header_declarations_exported(w2vmod):
+    names = declared_functions()
+    assert len(names) >= 19
+    lib = w2vmod.lib()
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in w2v.h but not exported"
+    assert set(names) == set(w2vmod.EXPORTS)
+
+
+def test_library_tables_equal_oracle_and_reference(w2vmod, orc):
+    e, n = w2vmod.sigmoid_tables()
+    oe, on = orc.exp_tables()
+    assert np.array_equal(e.view(np.uint32), oe.view(np.uint32))
+    assert np.array_equal(n.view(np.uint32), on.view(np.uint32))
+    assert hashlib.sha256(e.tobytes()).hexdigest() == \
+        "ca92d897d1437647e77391ec277693ca490df0e47d9a25b89db07840e8e1f56e"
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(layer_size=0), -5), (dict(layer_size=1025), -5), (dict(window=0), -5),
+    (dict(window=32), -5), (dict(negative=65), -5), (dict(vocab_size=0), -1),
+    (dict(msl=0), -1), (dict(streams=0), -1), (dict(sigmoid="table", mode="hogwild"), None),
+])
+def test_create_validates_before_touching_the_device(w2vmod, kw, code):
+    args = dict(vocab_size=10, layer_size=64, window=5, negative=5, msl=10, streams=1)
+    args.update(kw)
+    try:
+        tr = w2vmod.Trainer(**args)
+    except w2vmod.W2VError as e:
+        if code is None:
+            assert e.code == -2  # valid arguments, no GPU here
+        else:
+            assert e.code == code, str(e)
+        assert str(e)
+    else:
+        tr.close()  # a GPU is present and the arguments were valid
+        assert code is None

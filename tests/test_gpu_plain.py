@@ -1,0 +1,232 @@
+"""GPU parity of the FP32 plain kernel against the FP64 oracle (deterministic mode).
+
+Deltas are compared row-wise: for every row the oracle updates,
+    max_j |delta_gpu - delta_ref| <= TOL * max_j |delta_ref|,  TOL = 1e-5,
+after one sentence (or one short run of sentences), with the exact sigmoid on both
+sides. Rows the oracle leaves untouched must stay exactly zero on the GPU.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def row_rel_err(d_gpu, d_ref):
+    """Per-row infinity-norm relative error over rows the reference touched."""
+    scale = np.abs(d_ref).max(axis=1)
+    touched = scale > 0
+    err = np.abs(d_gpu.astype(np.float64) - d_ref).max(axis=1)
+    assert np.all(d_gpu[~touched] == 0), "GPU wrote rows the oracle never touched"
+    if not touched.any():
+        return 0.0
+    return float((err[touched] / scale[touched]).max())
+
+
+def model(V, d, rng, t_scale=0.5):
+    syn0 = ((rng.random((V, d)) - 0.5) / d).astype(np.float32)
+    syn1 = (rng.standard_normal((V, d)) * t_scale / np.sqrt(d)).astype(np.float32)
+    return syn0, syn1
+
+
+def gpu_delta(w2v, V, d, W, K, syn0, syn1, tokens, off, neg, alpha, sigmoid="exact", padded=False):
+    tr = w2v.Trainer(V, d, W, K, msl=max(1, int(np.diff(off).max())), sigmoid=sigmoid,
+                     mode="deterministic")
+    tr.upload(syn0, syn1)
+    tr.set_delta_mode(True)
+    if padded:
+        msl = tr.msl
+        n = len(off) - 1
+        sen = np.zeros((n, msl), np.int32)
+        ng = np.zeros((n, msl, max(K, 1)), np.int32)[:, :, :K]
+        ln = np.diff(off).astype(np.int32)
+        for s in range(n):
+            sen[s, :ln[s]] = tokens[off[s]:off[s + 1]]
+            if K:
+                ng[s, :ln[s]] = neg[off[s] * K:off[s + 1] * K].reshape(-1, K)
+        tr.submit_padded(sen, ng if K else None, ln, alpha)
+    else:
+        tr.submit_csr(tokens, off, neg if K else None, alpha)
+    d0, d1 = tr.delta_download()
+    tr.close()
+    return d0, d1
+
+
+def oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha, sig_mode=1):
+    s0 = syn0.astype(np.float64)
+    s1 = syn1.astype(np.float64)
+    a0, a1 = s0.copy(), s1.copy()
+    for s in range(len(off) - 1):
+        t = tokens[off[s]:off[s + 1]]
+        n = neg[off[s] * K:off[s + 1] * K] if K else None
+        a0, a1 = orc.sgns_sentence(a0, a1, W, K, t, n, float(np.float32(alpha[s])), sig_mode=sig_mode)
+    return a0 - s0, a1 - s1
+
+
+CASES = [
+    # V, d, W, K, lengths
+    (40, 64, 2, 3, [7]),
+    (40, 128, 5, 5, [1]),
+    (40, 128, 5, 5, [2]),
+    (40, 128, 5, 5, [11]),
+    (40, 128, 5, 5, [12]),
+    (40, 100, 5, 5, [37]),
+    (60, 300, 5, 5, [22]),
+    (30, 128, 1, 0, [9]),
+    (30, 64, 10, 15, [25]),
+    (200, 128, 5, 5, [130]),
+    (25, 32, 3, 2, [5, 0, 17, 3]),
+]
+
+
+@pytest.mark.parametrize("V,d,W,K,lens", CASES)
+def test_deterministic_deltas_match_fp64(orc, w2vmod, V, d, W, K, lens):
+    rng = np.random.default_rng(V * 1000 + d + 7 * W + K + len(lens))
+    syn0, syn1 = model(V, d, rng)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(0, V, off[-1]).astype(np.int32)
+    neg = rng.integers(0, V, off[-1] * K).astype(np.int32)
+    if K and off[-1]:
+        neg[0] = tokens[0]  # a negative equal to its centre
+    alpha = np.full(len(lens), 0.025, np.float32)
+    g0, g1 = gpu_delta(w2vmod, V, d, W, K, syn0, syn1, tokens, off, neg, alpha)
+    r0, r1 = oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha)
+    e0, e1 = row_rel_err(g0, r0), row_rel_err(g1, r1)
+    print(f"V={V} d={d} W={W} K={K} lens={lens}: syn0 rel {e0:.2e} syn1 rel {e1:.2e}", flush=True)
+    assert e0 <= TOL and e1 <= TOL
+
+
+def test_padded_equals_csr(orc, w2vmod):
+    V, d, W, K = 50, 128, 5, 5
+    rng = np.random.default_rng(3)
+    syn0, syn1 = model(V, d, rng)
+    lens = [13, 4, 29]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(0, V, off[-1]).astype(np.int32)
+    neg = rng.integers(0, V, off[-1] * K).astype(np.int32)
+    alpha = np.array([0.025, 0.02, 0.01], np.float32)
+    a = gpu_delta(w2vmod, V, d, W, K, syn0, syn1, tokens, off, neg, alpha)
+    b = gpu_delta(w2vmod, V, d, W, K, syn0, syn1, tokens, off, neg, alpha, padded=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_table_sigmoid_matches_oracle_table(orc, w2vmod):
+    """Table sigmoid: bucket-for-bucket agreement except rare FP32/FP64 edge straddles."""
+    V, d, W, K = 60, 128, 5, 5
+    passed = 0
+    trials = 12
+    for trial in range(trials):
+        rng = np.random.default_rng(100 + trial)
+        syn0, syn1 = model(V, d, rng, t_scale=2.0)
+        off = np.array([0, 20], np.int64)
+        tokens = rng.integers(0, V, 20).astype(np.int32)
+        neg = rng.integers(0, V, 20 * K).astype(np.int32)
+        alpha = np.array([0.025], np.float32)
+        g0, g1 = gpu_delta(w2vmod, V, d, W, K, syn0, syn1, tokens, off, neg, alpha, sigmoid="table")
+        r0, r1 = oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha, sig_mode=0)
+        ok = row_rel_err(g0, r0) <= TOL and row_rel_err(g1, r1) <= TOL
+        passed += ok
+    print(f"table sigmoid: {passed}/{trials} trials bucket-exact", flush=True)
+    assert passed >= trials - 2
+
+
+def test_device_sampling_deterministic(orc, w2vmod):
+    """Device subsampling + negatives + schedule, serialised, against the oracle replaying
+    the same keyed draws."""
+    V, d, W, K = 80, 64, 3, 4
+    seed, epochs = 11, 2
+    rng = np.random.default_rng(5)
+    syn0, syn1 = model(V, d, rng)
+    lens = [9, 15, 1, 30, 6]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    zipf = 1.0 / np.arange(1, V + 1)
+    tokens = rng.choice(V, off[-1], p=zipf / zipf.sum()).astype(np.int32)
+    cn = np.bincount(tokens, minlength=V).astype(np.uint64) + 1
+    T = int(off[-1])
+    subs = orc.subsample_probs(cn, 0.05, T)
+    uni = orc.unigram_table(cn, 10007)
+    tr = w2vmod.Trainer(V, d, W, K, msl=64, sigmoid="exact", mode="deterministic", seed=seed)
+    tr.upload(syn0, syn1)
+    tr.set_delta_mode(True)
+    tr.tables_upload(uni, subs)
+    tr.set_schedule(0.025, epochs, T)
+    tr.corpus_upload(tokens, off)
+    epoch = 1
+    tr.train_device(epoch, 0, len(lens))
+    g0, g1 = tr.delta_download()
+    st = tr.stats()
+    tr.close()
+    a0, a1 = syn0.astype(np.float64), syn1.astype(np.float64)
+    kept_total = 0
+    for s in range(len(lens)):
+        raw = tokens[off[s]:off[s + 1]]
+        keep = orc.device_keep(seed, epoch, raw, int(off[s]), subs)
+        kept = raw[keep]
+        kept_total += len(kept)
+        negs = orc.device_negatives(seed, epoch, int(off[s]), len(kept), K, uni).reshape(-1)
+        alpha = orc.lib().orc_learning_rate(0.025, epoch * T + int(off[s]), epochs, T)
+        a0, a1 = orc.sgns_sentence(a0, a1, W, K, kept, negs, float(np.float32(alpha)), sig_mode=1)
+    assert st["words_trained"] == kept_total and st["words_read"] == T
+    e0 = row_rel_err(g0, a0 - syn0.astype(np.float64))
+    e1 = row_rel_err(g1, a1 - syn1.astype(np.float64))
+    print(f"device sampling: kept {kept_total}/{T}, syn0 rel {e0:.2e} syn1 rel {e1:.2e}", flush=True)
+    assert e0 <= TOL and e1 <= TOL
+
+
+def test_heldout_loss_matches_oracle(orc, w2vmod):
+    V, d, W, K = 70, 96, 4, 5
+    rng = np.random.default_rng(9)
+    syn0, syn1 = model(V, d, rng, t_scale=1.0)
+    lens = [12, 1, 33, 7]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(0, V, off[-1]).astype(np.int32)
+    uni = rng.integers(1, V, 1000).astype(np.int32)
+    tr = w2vmod.Trainer(V, d, W, K, seed=21)
+    tr.upload(syn0, syn1)
+    tr.tables_upload(uni, None)
+    lg, pg = tr.heldout_loss(tokens, off)
+    tr.close()
+    lr, pr = orc.heldout_loss(syn0, syn1, W, K, tokens, off, uni, 21)
+    print(f"held-out loss gpu {lg:.9f} oracle {lr:.9f} pairs {pg}", flush=True)
+    assert pg == pr
+    assert abs(lg - lr) <= 1e-5 * abs(lr)
+
+
+def test_hogwild_runs_and_changes_model(orc, w2vmod):
+    V, d, W, K = 500, 128, 5, 5
+    rng = np.random.default_rng(17)
+    syn0 = orc.init_syn0(V, d)
+    syn1 = np.zeros((V, d), np.float32)
+    lens = rng.integers(1, 60, 4000)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(0, V, off[-1]).astype(np.int32)
+    neg = rng.integers(0, V, off[-1] * K).astype(np.int32)
+    alpha = np.full(len(lens), 0.025, np.float32)
+    tr = w2vmod.Trainer(V, d, W, K, msl=64, streams=2)
+    tr.upload(syn0, syn1)
+    half = len(lens) // 2
+    tr.submit_csr(tokens[:off[half]], off[:half + 1], neg[:off[half] * K], alpha[:half], stream=0)
+    off2 = off[half:] - off[half]
+    tr.submit_csr(tokens[off[half]:], off2, neg[off[half] * K:], alpha[half:], stream=1)
+    tr.sync()
+    s0, s1 = tr.download()
+    st = tr.stats()
+    tr.close()
+    assert np.isfinite(s0).all() and np.isfinite(s1).all()
+    assert st["words_trained"] == off[-1] and st["sentences"] == len(lens)
+    assert np.abs(s1).max() > 0 and np.abs(s0 - syn0).max() > 0
+
+
+def test_error_paths(w2vmod):
+    with pytest.raises(w2vmod.W2VError) as e:
+        w2vmod.Trainer(10, 2048, 5, 5)
+    assert e.value.code == -5
+    tr = w2vmod.Trainer(10, 64, 2, 2)
+    with pytest.raises(w2vmod.W2VError) as e:
+        tr.submit_csr(np.array([1, 10], np.int32), np.array([0, 2]), np.zeros(4, np.int32),
+                      np.array([0.025], np.float32))
+    assert e.value.code == -1 and "outside" in str(e.value)
+    with pytest.raises(w2vmod.W2VError):
+        tr.sync(stream=3)
+    tr.close()

@@ -1,0 +1,370 @@
+"""Pins of the oracle against the reference build, closed forms and invariants (CPU).
+
+The reference ships no tests or fixtures; what pins each oracle function here:
+  * sigmoid tables: the reference generator compiled from its own source (oracle/_ref)
+    and a sha256 of its output; the closed form sigma(x_i);
+  * table index, LCG, subsampling, initialisation, learning rate: closed forms computed
+    independently in numpy / Python integers;
+  * unigram table: the per-label share predicted by the reference's label shift;
+  * sentence update: hand closed form for one pair, the first-order (small-alpha) limit
+    and O(alpha^2) tile/sequential agreement, untouched-row invariants;
+  * sentence assembly: hand-traced LCG draws on a tiny corpus;
+  * held-out loss: closed form on a two-token corpus.
+"""
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_TABLE = os.path.join(ROOT, "oracle", "_ref", "expTable.h")
+# sha256 of the float32 bytes of expTable / nexpTable as printed by the reference's
+# generate_expTable_h.cpp (size 1000), compiled from its own source.
+SHA_EXP = "ca92d897d1437647e77391ec277693ca490df0e47d9a25b89db07840e8e1f56e"
+SHA_NEXP = "2f1e921443962b733f48fe2b59a3aa42adcdcd2bbaa676991559e09d32cb247f"
+M64 = (1 << 64) - 1
+A, C = 25214903917, 11
+
+
+def sigmoid(x):
+    return 1.0 / (1.0 + np.exp(-x))
+
+
+# ---------------------------------------------------------------- sigmoid tables
+def test_exp_tables_sha256_pin(orc):
+    e, n = orc.exp_tables()
+    assert hashlib.sha256(e.tobytes()).hexdigest() == SHA_EXP
+    assert hashlib.sha256(n.tobytes()).hexdigest() == SHA_NEXP
+
+
+def test_exp_tables_equal_reference_build(orc):
+    if not os.path.exists(REF_TABLE):
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    txt = open(REF_TABLE).read()
+    blocks = re.findall(r"\{([^}]*)\}", txt)
+    re_ = np.array([float(x) for x in blocks[0].split(",")], np.float32)
+    rn = np.array([float(x) for x in blocks[1].split(",")], np.float32)
+    e, n = orc.exp_tables()
+    assert np.array_equal(e.view(np.uint32), re_.view(np.uint32))
+    assert np.array_equal(n.view(np.uint32), rn.view(np.uint32))
+
+
+def test_exp_tables_closed_form(orc):
+    e, n = orc.exp_tables()
+    i = np.arange(1000)
+    x = (2 * i / 1000.0 - 1) * 6
+    inner = slice(1, 999)
+    assert np.abs(e[inner] - sigmoid(x[inner])).max() < 1.2e-7
+    assert e[0] == 0.0 and e[999] == 1.0 and n[0] == 1.0 and n[999] == 0.0
+    # nexp is the reversed table: nexp[j] = exp[999 - j] on the interior
+    assert np.array_equal(n[1:999], e[998:0:-1])
+    assert np.all(np.diff(e) > 0)
+
+
+def test_sigmoid_index(orc):
+    idx = orc.lib().orc_sigmoid_index
+    assert idx(-6.0) == 0 and idx(6.0) == 999 and idx(0.0) == 499
+    assert idx(float("nan")) == 0 and idx(float("inf")) == 999 and idx(float("-inf")) == 0
+    assert idx(-100.0) == 0 and idx(100.0) == 999
+    f = np.linspace(-7, 7, 20001).astype(np.float32)
+    got = np.array([idx(float(v)) for v in f])
+    ref = np.floor(np.clip((f.astype(np.float64) + 6) / 12, 0, 1) * 999).astype(int)
+    edges = (np.arange(1000) / 999.0) * 12 - 6
+    near = np.min(np.abs(f[:, None].astype(np.float64) - edges[None, :]), axis=1) < 1e-4
+    assert np.array_equal(got[~near], ref[~near])
+    assert np.all(np.abs(got[near] - ref[near]) <= 1)
+
+
+# ---------------------------------------------------------------- LCG
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_lcg_jump_ahead(orc, seed):
+    n = 1000
+    out = orc.lcg_stream(seed, n)
+    # x_k = A^k x_0 + C (A^k - 1)/(A - 1)  (mod 2^64), evaluated with Python integers
+    mod = 1 << 64
+    for k in (1, 2, 3, 10, 999, 1000):
+        ak = pow(A, k, mod * (A - 1))
+        geo = ((ak - 1) // (A - 1)) % mod
+        xk = (pow(A, k, mod) * seed + C * geo) % mod
+        assert int(out[k - 1]) == xk
+
+
+# ---------------------------------------------------------------- unigram table
+def test_unigram_label_shift(orc):
+    cn = np.array([40, 1000, 300, 200, 50, 7], np.uint64)
+    n = 200000
+    t = orc.unigram_table(cn, n)
+    p = cn.astype(np.float64) ** 0.75
+    p /= p.sum()
+    share = np.bincount(t, minlength=len(cn)) / n
+    assert share[0] == 0  # </s> is never drawn
+    assert abs(share[1] - p[0]) < 2e-5  # label 1 receives p_0
+    for i in range(2, len(cn) - 1):
+        assert abs(share[i] - p[i]) < 2e-5
+    assert abs(share[-1] - (p[-1] + p[1])) < 2e-5  # last label absorbs p_1
+
+
+def test_unigram_corrected(orc):
+    cn = np.array([40, 1000, 300, 200, 50, 7], np.uint64)
+    n = 200000
+    t = orc.unigram_table(cn, n, corrected=True)
+    p = cn.astype(np.float64) ** 0.75
+    p /= p.sum()
+    share = np.bincount(t, minlength=len(cn)) / n
+    assert np.abs(share - p).max() < 2e-5
+
+
+# ---------------------------------------------------------------- subsampling, init, lr
+def test_subsample_closed_form(orc):
+    cn = np.array([1, 5, 100, 10**6, 3 * 10**7], np.uint64)
+    T = 17 * 10**6
+    p = orc.subsample_probs(cn, 1e-4, T)
+    base = np.float32(1e-4) * np.float32(T)
+    ratio = base / cn.astype(np.float32)
+    ref = np.sqrt(ratio) + ratio
+    assert np.array_equal(p, ref.astype(np.float32))
+
+
+def test_init_closed_form(orc):
+    V, d = 7, 13
+    s = orc.init_syn0(V, d)
+    x = 1
+    ref = np.zeros(V * d, np.float32)
+    for i in range(V * d):
+        x = (x * A + C) & M64
+        ref[i] = (np.float32(x & 0xFFFF) / np.float32(65536.0) - np.float32(0.5)) / np.float32(d)
+    assert np.array_equal(s.reshape(-1), ref)
+    assert np.abs(s).max() <= 0.5 / d
+
+
+def test_learning_rate_closed_form(orc):
+    lr = orc.lib().orc_learning_rate
+    a0, E, T = 0.025, 3, 1000
+    for wca in (0, 1, 500, 1500, 2999, 3000, 5000):
+        ref = np.float32(a0) * (np.float32(1) - np.float32(wca) / np.float32(E * T + 1))
+        ref = max(ref, np.float32(a0) * np.float32(1e-4))
+        assert lr(a0, wca, E, T) == pytest.approx(float(ref), rel=1e-7)
+    assert lr(a0, 10**9, E, T) == pytest.approx(a0 * 1e-4, rel=1e-6)
+
+
+# ---------------------------------------------------------------- sentence update
+def rand_model(V, d, rng, ts=0.3):
+    s0 = ((rng.random((V, d)) - 0.5) / d).astype(np.float32)
+    s1 = (rng.standard_normal((V, d)) * ts).astype(np.float32)
+    return s0, s1
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+def test_one_pair_each_centre_closed_form(orc, tile):
+    """L=2, W=1, K=0: centre 0 trains target s0 against context 1, then centre 1 trains
+    target s1 against the untouched context 0."""
+    rng = np.random.default_rng(1)
+    V, d = 6, 8
+    s0, s1 = rand_model(V, d, rng)
+    sen = np.array([2, 4], np.int32)
+    a = 0.05
+    r0, r1 = orc.sgns_sentence(s0, s1, 1, 0, sen, None, a, sig_mode=1, tile=tile)
+    u = s0.astype(np.float64)
+    v = s1.astype(np.float64)
+    f = u[4] @ v[2]
+    g = a * (1 - sigmoid(f))
+    du4, dv2 = g * v[2], g * u[4]
+    f2 = u[2] @ v[4]
+    g2 = a * (1 - sigmoid(f2))
+    du2, dv4 = g2 * v[4], g2 * u[2]
+    e0, e1 = u.copy(), v.copy()
+    e0[4] += du4
+    e0[2] += du2
+    e1[2] += dv2
+    e1[4] += dv4
+    assert np.allclose(r0, e0, rtol=0, atol=1e-15)
+    assert np.allclose(r1, e1, rtol=0, atol=1e-15)
+
+
+def first_order_delta(s0, s1, W, K, sen, neg, a):
+    """Sum of per-pair SGNS gradients at the initial point (alpha -> 0 limit)."""
+    u = s0.astype(np.float64)
+    v = s1.astype(np.float64)
+    d0 = np.zeros_like(u)
+    d1 = np.zeros_like(v)
+    L = len(sen)
+    for w in range(L):
+        tg = [(sen[w], 1.0)] + [(neg[w * K + k], 0.0) for k in range(K)]
+        for c in range(max(0, w - W), min(L, w + W + 1)):
+            if c == w:
+                continue
+            for t, label in tg:
+                g = a * (label - sigmoid(u[sen[c]] @ v[t]))
+                d0[sen[c]] += g * v[t]
+                d1[t] += g * u[sen[c]]
+    return d0, d1
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+def test_first_order_limit(orc, tile):
+    rng = np.random.default_rng(2)
+    V, d, W, K, L = 20, 16, 3, 4, 12
+    s0, s1 = rand_model(V, d, rng)
+    sen = rng.integers(0, V, L).astype(np.int32)
+    neg = rng.integers(0, V, L * K).astype(np.int32)
+    errs = []
+    for a in (1e-3, 5e-4):
+        r0, r1 = orc.sgns_sentence(s0, s1, W, K, sen, neg, a, sig_mode=1, tile=tile)
+        f0, f1 = first_order_delta(s0, s1, W, K, sen, neg, a)
+        got = np.concatenate([(r0 - s0).ravel(), (r1 - s1).ravel()])
+        ref = np.concatenate([f0.ravel(), f1.ravel()])
+        errs.append(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    # the deviation from the first-order update is O(alpha): halving alpha halves it
+    assert errs[0] < 5e-2
+    assert 1.6 < errs[0] / errs[1] < 2.4
+
+
+def test_tile_vs_sequential_second_order(orc):
+    rng = np.random.default_rng(3)
+    V, d, W, K, L = 20, 16, 2, 3, 10
+    s0, s1 = rand_model(V, d, rng)
+    sen = rng.integers(0, V, L).astype(np.int32)
+    neg = rng.integers(0, V, L * K).astype(np.int32)
+    diffs = []
+    for a in (2e-3, 1e-3):
+        q0, q1 = orc.sgns_sentence(s0, s1, W, K, sen, neg, a, sig_mode=1, tile=0)
+        t0, t1 = orc.sgns_sentence(s0, s1, W, K, sen, neg, a, sig_mode=1, tile=1)
+        diffs.append(np.linalg.norm(q0 - t0) + np.linalg.norm(q1 - t1))
+    assert 3.2 < diffs[0] / diffs[1] < 4.8  # O(alpha^2)
+
+
+def test_untouched_rows_and_empty(orc):
+    rng = np.random.default_rng(4)
+    V, d, W, K = 30, 8, 2, 2
+    s0, s1 = rand_model(V, d, rng)
+    sen = np.array([1, 2, 3], np.int32)
+    neg = np.array([4, 5, 6, 7, 8, 9], np.int32)
+    r0, r1 = orc.sgns_sentence(s0, s1, W, K, sen, neg, 0.025)
+    assert np.array_equal(r0[10:], s0[10:]) and np.array_equal(r1[10:], s1[10:])
+    assert np.array_equal(r0[4:10], s0[4:10])  # negatives never touch syn0
+    e0, e1 = orc.sgns_sentence(s0, s1, W, K, np.zeros(0, np.int32), None, 0.025)
+    assert np.array_equal(e0, s0) and np.array_equal(e1, s1)
+
+
+def test_bf16_emulation_bounded(orc):
+    rng = np.random.default_rng(5)
+    V, d, W, K, L = 20, 32, 3, 4, 14
+    s0, s1 = rand_model(V, d, rng)
+    sen = rng.integers(0, V, L).astype(np.int32)
+    neg = rng.integers(0, V, L * K).astype(np.int32)
+    a0, a1 = orc.sgns_sentence(s0, s1, W, K, sen, neg, 0.025, tile=1, bf16=0)
+    b0, b1 = orc.sgns_sentence(s0, s1, W, K, sen, neg, 0.025, tile=1, bf16=1)
+    da = np.concatenate([(a0 - s0).ravel(), (a1 - s1).ravel()])
+    db = np.concatenate([(b0 - s0).ravel(), (b1 - s1).ravel()])
+    rel = np.linalg.norm(da - db) / np.linalg.norm(da)
+    assert 1e-5 < rel < 2e-2  # bf16 operands perturb, but only at the bf16 scale
+
+
+# ---------------------------------------------------------------- sentence assembly
+def lcg_py(x):
+    return (x * A + C) & M64
+
+
+def test_assembly_hand_traced(orc):
+    """Three corpus lines with </s> kept (-cache-eos), no subsampling, K=2, msl 4:
+    GPU sentences are the lines (split at 4 kept tokens), negatives are consecutive
+    LCG draws from seed = thread id, and the last sentence of the epoch is dropped."""
+    # createGlobalStream with msl=4 and cache_eos: sentences start at 0,4,7,11 and the
+    # stream ends with an extra </s>
+    stream = np.array([5, 6, 7, 0, 8, 9, 0, 3, 4, 5, 6, 7, 0, 0], np.int32)
+    starts = np.array([0, 4, 7, 11], np.int64)
+    uni = np.arange(100, dtype=np.int32) % 10
+    tok, ng, off, al, st = orc.assemble_epoch(stream, starts, 0, 4, 4, 2, 0.0, None, uni, 0.025, 1,
+                                              13, state=orc.ThreadState(3, 0, 0, 0, 0.025))
+    lens = np.diff(off)
+    # lines: [5,6,7] [8,9] [3,4,5,6 | 7] ; the final GPU sentence is dropped at epoch end
+    assert list(lens[:3]) == [3, 2, 4]
+    assert list(tok[:9]) == [5, 6, 7, 8, 9, 3, 4, 5, 6]
+    x = 3
+    exp_neg = []
+    for _ in range(9 * 2):
+        x = lcg_py(x)
+        exp_neg.append(int(uni[(x >> 16) % 100]))
+    assert list(ng[:18]) == exp_neg
+    assert list(lens) == [3, 2, 4, 0]  # the trailing [7] is dropped at epoch end
+    assert st.word_count_actual == 13  # every token read, including </s>
+    assert np.all(al == np.float32(0.025))
+
+
+def test_assembly_subsampling_draw_order(orc):
+    """With subsampling, each token read takes one LCG step before the K negative draws."""
+    stream = np.array([1, 2, 1, 2, 0, 1, 0, 0], np.int32)
+    starts = np.array([0, 5], np.int64)
+    uni = np.array([1, 2, 3, 4], np.int32)
+    subs = np.array([1.0, 0.3, 2.0], np.float32)  # word 1 kept iff draw <= 0.3
+    tok, ng, off, al, st = orc.assemble_epoch(stream, starts, 0, 2, 8, 1, 1e-3, subs, uni, 0.025, 1,
+                                              4, state=orc.ThreadState(0, 0, 0, 0, 0.025))
+    x = 0
+    kept, negs = [], []
+    for w in [1, 2, 1, 2]:
+        x = lcg_py(x)
+        if subs[w] < np.float32(x & 0xFFFF) / np.float32(65536.0):
+            continue
+        kept.append(w)
+        x = lcg_py(x)
+        negs.append(int(uni[(x >> 16) % 4]))
+    assert list(tok[:len(kept)]) == kept and list(ng[:len(kept)]) == negs
+
+
+# ---------------------------------------------------------------- device keyed draws
+def test_keyed_stream(orc):
+    L = orc.lib()
+    m = L.orc_mix64
+    # splitmix64 reference values of the first outputs from state 0
+    assert m(0) == 0xE220A8397B1DCDAF
+    assert L.orc_key(7, 3, 5) == m(m(m(7) ^ 3) ^ 5)
+    uni = np.arange(1000, dtype=np.int32)
+    negs = orc.device_negatives(1, 0, 0, 4000, 5, uni).ravel()
+    counts = np.bincount(negs // 100, minlength=10)
+    assert counts.min() > 1700 and counts.max() < 2300  # uniform table -> uniform draws
+
+
+# ---------------------------------------------------------------- synthetic corpus
+def test_synth_corpus_properties(orc):
+    cfg = orc.synth_cfg(V=2001, clusters=8, len_median=22, len_max=200, seed=1)
+    tok, off = orc.synth_corpus(cfg, 3000)
+    lens = np.diff(off)
+    assert lens.min() >= 1 and lens.max() <= 200
+    assert 19 <= np.median(lens) <= 25
+    assert tok.min() >= 1 and tok.max() <= 2000
+    freq = np.bincount(tok, minlength=2001)
+    # mixture of Zipf(1) over the vocabulary and over each cluster's members: the
+    # rank-frequency slope stays close to -1
+    r = np.arange(5, 500)
+    slope = np.polyfit(np.log(r), np.log(freq[r] + 0.5), 1)[0]
+    assert -1.25 < slope < -0.75
+    # consecutive tokens share a cluster far more often than chance (1/8)
+    import ctypes
+
+    clus = np.array([orc.lib().orc_synth_cluster_of(ctypes.byref(cfg), int(w)) for w in range(2001)])
+    same = []
+    for s in range(3000):
+        t = tok[off[s]:off[s + 1]]
+        if len(t) > 1:
+            same.append(np.mean(clus[t[1:]] == clus[t[:-1]]))
+    assert np.mean(same) > 0.5
+
+
+# ---------------------------------------------------------------- held-out loss
+def test_heldout_loss_closed_form(orc):
+    rng = np.random.default_rng(6)
+    V, d = 5, 4
+    s0, s1 = rand_model(V, d, rng, ts=0.7)
+    tokens = np.array([1, 3], np.int32)
+    off = np.array([0, 2], np.int64)
+    uni = np.array([2], np.int32)  # every negative is word 2
+    loss, pairs = orc.heldout_loss(s0, s1, 1, 1, tokens, off, uni, 1)
+    u = s0.astype(np.float64)
+    v = s1.astype(np.float64)
+    nls = lambda x: np.log1p(np.exp(-x))
+    l1 = nls(u[3] @ v[1]) + nls(-(u[3] @ v[2]))  # centre 1, context 3
+    l2 = nls(u[1] @ v[3]) + nls(-(u[1] @ v[2]))  # centre 3, context 1
+    assert pairs == 2
+    assert loss == pytest.approx((l1 + l2) / 2, rel=1e-12)
