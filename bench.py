@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""SGNS training throughput on one MI355X.
+
+Workload (the headline configuration): cluster-Markov Zipf corpus, vocabulary 1M
+(64 clusters, Zipf s=1.0, stay 0.9, 80% in-cluster draws), about 2.0B tokens in
+sentences of log-normal length (median 22, sigma 1.0, max 1000), d=300, window 5,
+5 negatives, subsampling 1e-4, learning rate 0.025 with linear decay, one epoch.
+The corpus, the unigram table and both matrices are resident in HBM before timing.
+
+A step is one device pass over one twentieth of the epoch's sentences (device
+subsampling, negative draws and the update kernel). Words/sec follows the reference
+definition (FULL-W2V-driver.cu:450): corpus tokens read, including dropped ones,
+over wall time.
+
+Every kernel variant the library offers is run on the same corpus; the headline
+value is the fastest variant's words/sec; held-out SGNS loss (FP64, 20k held-out
+sentences from a second seed) and planted-cluster nearest-neighbour recovery are
+reported for each. The CPU baseline is the oracle's multithreaded FP32 Hogwild
+trainer timed on a bounded prefix of the same corpus on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "full-w2v_amd"))
+
+METRIC = "trained words/sec at d=300,w=5,neg=5: BF16-MFMA vs FP32 plain, equal held-out loss"
+HBM_PEAK_GBS = 8000.0
+CHUNKS = 20
+WL = dict(V=1_000_000, d=300, W=5, K=5, sample=1e-4, alpha=0.025, n_sent=55_000_000,
+          clusters=64, median=22, len_max=1000, sigma=1.0, zipf=1.0, stay=0.9, in_cluster=0.8,
+          seed=1, heldout_seed=2, heldout_sent=20_000, unigram=100_000_000)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_per_word(d, K, e):
+    """Algorithmic HBM bytes per kept centre word: 1+K target rows read and written,
+    one context row read and written once per lifetime, 4(1+K) index bytes."""
+    return (2 * (1 + K) + 2) * d * e + 4 * (1 + K)
+
+
+def nn_recovery(syn0, labels, words, k=10):
+    """Fraction of each word's k cosine nearest neighbours (among `words`) that share
+    its planted cluster."""
+    x = syn0[words].astype(np.float64)
+    x /= np.linalg.norm(x, axis=1, keepdims=True) + 1e-12
+    sim = x @ x.T
+    np.fill_diagonal(sim, -np.inf)
+    nn = np.argpartition(-sim, k, axis=1)[:, :k]
+    lab = labels[words]
+    return float(np.mean(lab[nn] == lab[:, None]))
+
+
+def variants(w2v):
+    out = [("plain", "fp32")]
+    try:
+        t = w2v.Trainer(16, 64, 2, 2, storage="bf16", kernel="mfma")
+        t.close()
+        out.insert(0, ("mfma", "bf16"))
+    except w2v.W2VError:
+        pass
+    return out
+
+
+def run_variant(w2v, kernel, storage, steps, warmup, host):
+    V, d, W, K = WL["V"], WL["d"], WL["W"], WL["K"]
+    tr = w2v.Trainer(V, d, W, K, msl=WL["len_max"], storage=storage, kernel=kernel,
+                     sigmoid="table", mode="hogwild", seed=WL["seed"])
+    cfg = w2v.synth_config(WL["clusters"], WL["median"], WL["len_max"], WL["zipf"], WL["stay"],
+                           WL["in_cluster"], WL["sigma"], WL["seed"])
+    t = time.perf_counter()
+    tr.synth_corpus(cfg, WL["n_sent"])
+    nt, ns = tr.corpus_info()
+    log(f"[{kernel}/{storage}] corpus: {ns} sentences, {nt} tokens ({time.perf_counter() - t:.1f}s)")
+    if host.get("counts") is None:
+        cn = tr.corpus_counts()
+        cn[0] = ns  # </s>: one per line, as LearnVocabFromTrainFile counts newlines
+        host["counts"] = cn
+        host["unigram"] = w2v.unigram_table(cn, WL["unigram"])
+        host["subs"] = w2v.subsample_probs(cn, WL["sample"], nt)
+        host["tokens"], host["nt"], host["ns"] = None, nt, ns
+    tr.tables_upload(host["unigram"], host["subs"])
+    epochs = max(1, math.ceil((steps + warmup) / CHUNKS))
+    tr.set_schedule(WL["alpha"], epochs, nt)
+    tr.init_model()
+    if host.get("cpu_sample") is None:
+        host["cpu_sample"] = tr.corpus_download(0, 600_000)
+    bounds = [round(i * ns / CHUNKS) for i in range(CHUNKS + 1)]
+
+    def step(i):
+        e, c = divmod(i, CHUNKS)
+        tr.train_device(e, bounds[c], bounds[c + 1])
+
+    for i in range(warmup):
+        step(i)
+    tr.sync()
+    ms0, n0 = tr.kernel_time(0)
+    st0 = tr.stats()
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        step(i)
+        tr.sync()  # one step in flight; also paces the progress lines
+        if (i - warmup) % 5 == 4:
+            log(f"[{kernel}/{storage}] step {i - warmup + 1}/{steps} {time.perf_counter() - t0:.1f}s")
+    t1 = time.perf_counter()
+    ms1, n1 = tr.kernel_time(0)
+    st1 = tr.stats()
+    secs = t1 - t0
+    words = st1["words_read"] - st0["words_read"]
+    kept = st1["words_trained"] - st0["words_trained"]
+    launches = n1 - n0
+    avg_ms = (ms1 - ms0) / max(launches, 1)
+    e = 2 if storage == "bf16" else 4
+    bpw = bytes_per_word(d, K, e)
+    per_launch = kept / max(launches, 1) * bpw
+    achieved = per_launch / (avg_ms / 1e3) / 1e9
+    log(f"[{kernel}/{storage}] {words / secs / 1e6:.1f} M words/s, kept {kept / words:.3f}, "
+        f"kernel avg {avg_ms:.1f} ms, {achieved:.0f} GB/s algorithmic")
+    # quality: held-out loss and planted-cluster recovery
+    s0, _ = tr.download()
+    hcfg = w2v.synth_config(WL["clusters"], WL["median"], WL["len_max"], WL["zipf"], WL["stay"],
+                            WL["in_cluster"], WL["sigma"], WL["seed"])
+    hcfg.seed = WL["heldout_seed"]
+    if host.get("heldout") is None:
+        tr.synth_corpus(hcfg, WL["heldout_sent"])
+        host["heldout"] = tr.corpus_download(0, WL["heldout_sent"])
+    loss, pairs = tr.heldout_loss(*host["heldout"])
+    if host.get("labels") is None:
+        host["labels"] = w2v.synth_cluster_of(WL["seed"], WL["clusters"], np.arange(V))
+    probe = np.arange(2, 2002)
+    rec = nn_recovery(s0, host["labels"], probe)
+    tr.close()
+    log(f"[{kernel}/{storage}] held-out loss {loss:.6f} over {pairs} pairs, NN recovery {rec:.4f}")
+    return dict(kernel=kernel, storage=storage, words_per_sec=words / secs, seconds=secs,
+                words=int(words), kept_words=int(kept), launches=int(launches),
+                kernel_avg_ms=avg_ms, bytes_per_word=bpw, bytes_per_launch=per_launch,
+                achieved_gbs=achieved, heldout_loss=loss, heldout_pairs=int(pairs),
+                nn_recovery=rec, ms_per_step=secs / max(steps, 1) * 1e3)
+
+
+def cpu_baseline(host, target_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+
+    tok, off = host["cpu_sample"]
+    threads = min(16, len(os.sched_getaffinity(0)))
+    V, d, W, K = WL["V"], WL["d"], WL["W"], WL["K"]
+
+    def to_stream(ns):
+        # reference corpus layout (improved_driver_functions.cpp:244-281 with -cache-eos):
+        # each line followed by </s>, sentence starts into one stream ending in 0
+        lens = np.diff(off[:ns + 1])
+        starts = off[:ns] + np.arange(ns)
+        stream = np.zeros(off[ns] + ns + 1, np.int32)
+        for s in range(ns):
+            stream[starts[s]:starts[s] + lens[s]] = tok[off[s]:off[s + 1]]
+        return stream, starts.astype(np.int64), int(off[ns])
+
+    syn0 = orc.init_syn0(V, d)
+    syn1 = np.zeros((V, d), np.float32)
+    ns = min(len(off) - 1, 20_000)
+    stream, starts, nw = to_stream(ns)
+    wps, words, secs = orc.train_hogwild(stream, starts, nw, host["unigram"], host["subs"], syn0, syn1,
+                                         d, W, K, WL["len_max"], 1, threads, WL["alpha"], WL["sample"])
+    want = int(min(len(off) - 1, max(ns, ns * target_s / max(secs, 1e-3))))
+    stream, starts, nw = to_stream(want)
+    wps, words, secs = orc.train_hogwild(stream, starts, nw, host["unigram"], host["subs"], syn0, syn1,
+                                         d, W, K, WL["len_max"], 1, threads, WL["alpha"], WL["sample"])
+    log(f"[cpu] {wps / 1e6:.3f} M words/s on {threads} threads ({words} words, {secs:.1f}s)")
+    return dict(value=wps, unit="words/s", cores=threads, kind="port",
+                sample=f"first {want} sentences ({words} tokens) of the same corpus, 1 epoch, "
+                       f"FP32 Hogwild, table sigmoid")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=19)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--variant", default="all", help="all | plain | mfma")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import w2v
+
+    host = {}
+    vs = variants(w2v)
+    if args.variant != "all":
+        vs = [v for v in vs if v[0] == args.variant]
+    results = {}
+    for kernel, storage in vs:
+        results[kernel] = run_variant(w2v, kernel, storage, args.steps, args.warmup, host)
+    cpu = None if args.no_cpu else cpu_baseline(host)
+    best = max(results.values(), key=lambda r: r["words_per_sec"])
+    line = {
+        "metric": METRIC,
+        "value": best["words_per_sec"],
+        "unit": "words/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if best["storage"] == "bf16" else "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "cluster-Markov Zipf corpus, V=1M, 64 clusters, lognormal sentences "
+                        "(median 22, max 1000), d=300 window 5 negative 5 sample 1e-4, 1 epoch",
+            "vocab": WL["V"], "tokens": int(host["nt"]), "sentences": int(host["ns"]),
+            "layer_size": WL["d"], "window": WL["W"], "negative": WL["K"],
+            "kernel": best["kernel"], "storage": best["storage"], "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": best["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": best["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "variants": results,
+    }
+    if "plain" in results and "mfma" in results:
+        line["speedup_mfma_vs_plain"] = results["mfma"]["words_per_sec"] / results["plain"]["words_per_sec"]
+        line["heldout_loss_rel_diff"] = (results["mfma"]["heldout_loss"] - results["plain"]["heldout_loss"]) \
+            / results["plain"]["heldout_loss"]
+    if cpu:
+        line["gpu_vs_cpu"] = best["words_per_sec"] / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

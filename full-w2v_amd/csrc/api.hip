@@ -9,10 +9,15 @@
 #include <vector>
 
 #include "common.h"
+#include "synth_tables.h"
 #include "w2v.h"
 
 namespace w2v {
 hipError_t launch_plain(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st);
+hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st);
+hipError_t occupancy_mfma(int P, int K, int waves_per_block, size_t lds, int* blocks_per_cu);
+size_t mfma_lds_per_wave(int W, int K, int P, bool delta);
+bool mfma_supported(int d, int W, int K);
 hipError_t occupancy_plain(int P, int waves_per_block, size_t lds, int* blocks_per_cu);
 size_t plain_lds_per_wave(int W, int P);
 hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
@@ -22,7 +27,8 @@ hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t
                           int32_t* flags, int32_t* pos, int32_t* kept, int64_t* kept_off,
                           void* temp, size_t temp_bytes, int64_t tok_begin, int64_t n,
                           hipStream_t st);
-hipError_t launch_heldout_loss(const float* syn0, const float* syn1, int P, int W, int K,
+hipError_t launch_heldout_loss(const float* syn0, const float* syn1, const uint16_t* bsyn0,
+                               const uint16_t* bsyn1, int P, int W, int K,
                                const int32_t* tokens, const int64_t* off, int64_t n_sent,
                                const int32_t* unigram, int64_t n_unigram, uint64_t mixed_seed,
                                double* acc, unsigned long long* pairs, int num_cu, hipStream_t st);
@@ -66,8 +72,12 @@ struct w2v_ctx {
   int num_cu = 0;
   int waves_per_block = 4;
   int blocks = 0;  // persistent grid of the update kernel
-  float* syn0 = nullptr;
+  float* syn0 = nullptr;  // FP32 storage
   float* syn1 = nullptr;
+  uint16_t* bsyn0 = nullptr;  // bf16 storage
+  uint16_t* bsyn1 = nullptr;
+  int cfg_wpb[2] = {0, 0};  // update-kernel launch shape, [delta mode]
+  int cfg_blocks[2] = {0, 0};
   float* dsyn0 = nullptr;
   float* dsyn1 = nullptr;
   int delta_mode = 0;
@@ -180,6 +190,8 @@ UpdateArgs base_args(w2v_ctx* c) {
   UpdateArgs a{};
   a.syn0 = c->syn0;
   a.syn1 = c->syn1;
+  a.bsyn0 = c->bsyn0;
+  a.bsyn1 = c->bsyn1;
   if (c->delta_mode) {
     a.out0 = c->dsyn0;
     a.out1 = c->dsyn1;
@@ -210,7 +222,25 @@ int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a) {
     return fail(c, W2V_ERR_ARG, "at most 2^31-1 sentences per launch (got %lld)", (long long)a.n_sent);
   a.queue = s.queue;
   HIP_TRY(c, launch_zero_u32(s.queue, 1, s.st));
-  int wpb = c->waves_per_block, blocks = c->blocks;
+  const int dm = c->delta_mode ? 1 : 0;
+  if (!c->cfg_blocks[dm]) {
+    // waves per block and persistent grid from the per-wave LDS ring and occupancy
+    const bool mf = c->p.kernel == W2V_KERNEL_MFMA;
+    const size_t lw = mf ? mfma_lds_per_wave(c->p.window, c->p.negative, c->P, dm)
+                         : plain_lds_per_wave(c->p.window, c->P);
+    const size_t lmax = 160 * 1024;
+    if (lw > lmax)
+      return fail(c, W2V_ERR_UNSUPPORTED, "per-wave LDS of %zu bytes exceeds 160 KiB", lw);
+    int wpb = 4;
+    while (wpb > 1 && lw * wpb > lmax) wpb--;
+    int bpc = 0;
+    hipError_t oe = mf ? occupancy_mfma(c->P, c->p.negative, wpb, lw * wpb, &bpc)
+                       : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
+    if (oe != hipSuccess || bpc < 1) bpc = 1;
+    c->cfg_wpb[dm] = wpb;
+    c->cfg_blocks[dm] = bpc * c->num_cu;
+  }
+  int wpb = c->cfg_wpb[dm], blocks = c->cfg_blocks[dm];
   if (c->p.mode == W2V_MODE_DETERMINISTIC) {
     wpb = 1;
     blocks = 1;
@@ -219,7 +249,8 @@ int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a) {
   HIP_TRY(c, hipEventCreate(&e0));
   HIP_TRY(c, hipEventCreate(&e1));
   HIP_TRY(c, hipEventRecord(e0, s.st));
-  HIP_TRY(c, launch_plain(a, wpb, blocks, s.st));
+  if (c->p.kernel == W2V_KERNEL_MFMA) HIP_TRY(c, launch_mfma(a, wpb, blocks, s.st));
+  else HIP_TRY(c, launch_plain(a, wpb, blocks, s.st));
   HIP_TRY(c, hipEventRecord(e1, s.st));
   s.pending.emplace_back(e0, e1);
   s.launches++;
@@ -252,10 +283,15 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
     return fail(nullptr, W2V_ERR_UNSUPPORTED, "negative %d outside 0..64", p.negative);
   if (p.max_sentence_length < 1) return fail(nullptr, W2V_ERR_ARG, "max_sentence_length must be >= 1");
   if (p.streams < 1 || p.streams > 64) return fail(nullptr, W2V_ERR_ARG, "streams must be in 1..64");
-  if (p.storage != W2V_STORAGE_FP32)
-    return fail(nullptr, W2V_ERR_UNSUPPORTED, "storage %d not available in this build", p.storage);
-  if (p.kernel != W2V_KERNEL_PLAIN)
-    return fail(nullptr, W2V_ERR_UNSUPPORTED, "kernel %d not available in this build", p.kernel);
+  if (!((p.kernel == W2V_KERNEL_PLAIN && p.storage == W2V_STORAGE_FP32) ||
+        (p.kernel == W2V_KERNEL_MFMA && p.storage == W2V_STORAGE_BF16)))
+    return fail(nullptr, W2V_ERR_UNSUPPORTED,
+                "kernel/storage pair (%d,%d) not available: plain runs on FP32 rows, mfma on bf16 rows",
+                p.kernel, p.storage);
+  if (p.kernel == W2V_KERNEL_MFMA && !mfma_supported(p.layer_size, p.window, p.negative))
+    return fail(nullptr, W2V_ERR_UNSUPPORTED,
+                "the mfma kernel needs d <= 512, 2W+1 <= 16 and K+1 <= 16 (d=%d W=%d K=%d)",
+                p.layer_size, p.window, p.negative);
   if (p.sigmoid != W2V_SIGMOID_TABLE && p.sigmoid != W2V_SIGMOID_EXACT)
     return fail(nullptr, W2V_ERR_ARG, "unknown sigmoid mode %d", p.sigmoid);
   if (p.mode != W2V_MODE_HOGWILD && p.mode != W2V_MODE_DETERMINISTIC)
@@ -282,23 +318,17 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
     return bail(fail(c, W2V_ERR_HIP, "hipGetDeviceProperties failed"));
   c->num_cu = prop.multiProcessorCount;
 
-  const size_t lds_wave = plain_lds_per_wave(p.window, c->P);
-  const size_t lds_max = 160 * 1024;
-  if (lds_wave > lds_max)
+  if (p.kernel == W2V_KERNEL_PLAIN && plain_lds_per_wave(p.window, c->P) > 160 * 1024)
     return bail(fail(c, W2V_ERR_UNSUPPORTED,
                      "window ring of %zu bytes exceeds the 160 KiB LDS of a CU (d=%d, W=%d)",
-                     lds_wave, p.layer_size, p.window));
-  int wpb = 4;
-  while (wpb > 1 && lds_wave * wpb > lds_max) wpb--;
-  c->waves_per_block = wpb;
-  int bpc = 0;
-  if (occupancy_plain(c->P, wpb, lds_wave * wpb, &bpc) != hipSuccess || bpc < 1) bpc = 1;
-  c->blocks = bpc * c->num_cu;
-
-  const size_t rows = (size_t)p.vocab_size * c->P * sizeof(float);
-  if (hipMalloc(&c->syn0, rows) != hipSuccess || hipMalloc(&c->syn1, rows) != hipSuccess)
+                     plain_lds_per_wave(p.window, c->P), p.layer_size, p.window));
+  const bool bf = p.storage == W2V_STORAGE_BF16;
+  const size_t rows = (size_t)p.vocab_size * c->P * (bf ? 2 : 4);
+  void** m0 = bf ? (void**)&c->bsyn0 : (void**)&c->syn0;
+  void** m1 = bf ? (void**)&c->bsyn1 : (void**)&c->syn1;
+  if (hipMalloc(m0, rows) != hipSuccess || hipMalloc(m1, rows) != hipSuccess)
     return bail(fail(c, W2V_ERR_NOMEM, "model allocation of 2 x %zu bytes failed", rows));
-  if (hipMemset(c->syn0, 0, rows) != hipSuccess || hipMemset(c->syn1, 0, rows) != hipSuccess)
+  if (hipMemset(*m0, 0, rows) != hipSuccess || hipMemset(*m1, 0, rows) != hipSuccess)
     return bail(fail(c, W2V_ERR_HIP, "model clear failed"));
   float e[kExpTableSize], ne[kExpTableSize];
   make_tables(e, ne);
@@ -337,7 +367,7 @@ int w2v_destroy(w2v_ctx* c) {
     if (s.queue) hipFree(s.queue);
     if (s.st) hipStreamDestroy(s.st);
   }
-  for (void* q : {(void*)c->syn0, (void*)c->syn1, (void*)c->dsyn0, (void*)c->dsyn1, (void*)c->d_exp,
+  for (void* q : {(void*)c->syn0, (void*)c->syn1, (void*)c->bsyn0, (void*)c->bsyn1, (void*)c->dsyn0, (void*)c->dsyn1, (void*)c->d_exp,
                   (void*)c->d_nexp, (void*)c->d_counters, (void*)c->d_unigram, (void*)c->d_subs,
                   (void*)c->d_corpus, (void*)c->d_off})
     if (q) hipFree(q);
@@ -374,13 +404,52 @@ int w2v_sync(w2v_ctx* c, int32_t stream) {
   return drain_timing(c, c->ss[stream]);
 }
 
+static uint16_t to_bf16(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+static int copy_matrix(w2v_ctx* c, void* dev, const float* host, bool up) {
+  const size_t d = c->p.layer_size, V = c->p.vocab_size;
+  if (c->p.storage == W2V_STORAGE_FP32) {
+    const size_t pitch = c->P * sizeof(float);
+    if (up) HIP_TRY(c, hipMemcpy2D(dev, pitch, host, d * 4, d * 4, V, hipMemcpyHostToDevice));
+    else HIP_TRY(c, hipMemcpy2D((void*)host, d * 4, dev, pitch, d * 4, V, hipMemcpyDeviceToHost));
+    return W2V_OK;
+  }
+  // bf16 rows: convert through a host staging buffer, in row blocks
+  const size_t blk = 65536;
+  std::vector<uint16_t> buf(blk * c->P, 0);
+  for (size_t r0 = 0; r0 < V; r0 += blk) {
+    const size_t nr = V - r0 < blk ? V - r0 : blk;
+    uint16_t* dp = (uint16_t*)dev + r0 * c->P;
+    if (up) {
+      for (size_t r = 0; r < nr; r++)
+        for (size_t j = 0; j < d; j++) buf[r * c->P + j] = to_bf16(host[(r0 + r) * d + j]);
+      HIP_TRY(c, hipMemcpy(dp, buf.data(), nr * c->P * 2, hipMemcpyHostToDevice));
+    } else {
+      HIP_TRY(c, hipMemcpy(buf.data(), dp, nr * c->P * 2, hipMemcpyDeviceToHost));
+      float* h = (float*)host;
+      for (size_t r = 0; r < nr; r++)
+        for (size_t j = 0; j < d; j++) {
+          const uint32_t u = (uint32_t)buf[r * c->P + j] << 16;
+          memcpy(&h[(r0 + r) * d + j], &u, 4);
+        }
+    }
+  }
+  return W2V_OK;
+}
+
 int w2v_model_upload(w2v_ctx* c, const float* syn0, const float* syn1neg) {
   if (!c) return W2V_ERR_ARG;
   int rc = w2v_sync_all(c);
   if (rc) return rc;
-  const size_t d = c->p.layer_size, V = c->p.vocab_size, pitch = c->P * sizeof(float);
-  if (syn0) HIP_TRY(c, hipMemcpy2D(c->syn0, pitch, syn0, d * sizeof(float), d * sizeof(float), V, hipMemcpyHostToDevice));
-  if (syn1neg) HIP_TRY(c, hipMemcpy2D(c->syn1, pitch, syn1neg, d * sizeof(float), d * sizeof(float), V, hipMemcpyHostToDevice));
+  const bool bf = c->p.storage == W2V_STORAGE_BF16;
+  if (syn0 && (rc = copy_matrix(c, bf ? (void*)c->bsyn0 : (void*)c->syn0, syn0, true))) return rc;
+  if (syn1neg && (rc = copy_matrix(c, bf ? (void*)c->bsyn1 : (void*)c->syn1, syn1neg, true))) return rc;
   return W2V_OK;
 }
 
@@ -388,9 +457,9 @@ int w2v_model_download(w2v_ctx* c, float* syn0, float* syn1neg) {
   if (!c) return W2V_ERR_ARG;
   int rc = w2v_sync_all(c);
   if (rc) return rc;
-  const size_t d = c->p.layer_size, V = c->p.vocab_size, pitch = c->P * sizeof(float);
-  if (syn0) HIP_TRY(c, hipMemcpy2D(syn0, d * sizeof(float), c->syn0, pitch, d * sizeof(float), V, hipMemcpyDeviceToHost));
-  if (syn1neg) HIP_TRY(c, hipMemcpy2D(syn1neg, d * sizeof(float), c->syn1, pitch, d * sizeof(float), V, hipMemcpyDeviceToHost));
+  const bool bf = c->p.storage == W2V_STORAGE_BF16;
+  if (syn0 && (rc = copy_matrix(c, bf ? (void*)c->bsyn0 : (void*)c->syn0, syn0, false))) return rc;
+  if (syn1neg && (rc = copy_matrix(c, bf ? (void*)c->bsyn1 : (void*)c->syn1, syn1neg, false))) return rc;
   return W2V_OK;
 }
 
@@ -650,7 +719,7 @@ int w2v_heldout_loss(w2v_ctx* c, const int32_t* tokens, const int64_t* off, int6
   if (e == hipSuccess) e = hipMemset(acc, 0, 8);
   if (e == hipSuccess) e = hipMemset(np, 0, 8);
   if (e == hipSuccess)
-    e = launch_heldout_loss(c->syn0, c->syn1, c->P, c->p.window, c->p.negative, dt, doff, n_sent,
+    e = launch_heldout_loss(c->syn0, c->syn1, c->bsyn0, c->bsyn1, c->P, c->p.window, c->p.negative, dt, doff, n_sent,
                             c->d_unigram, c->n_unigram, mix64(c->p.seed), acc, np, c->num_cu, c->ss[0].st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->ss[0].st);
   double hacc = 0.0;
@@ -661,6 +730,202 @@ int w2v_heldout_loss(w2v_ctx* c, const int32_t* tokens, const int64_t* off, int6
   if (e != hipSuccess) return fail(c, W2V_ERR_HIP, "held-out loss failed: %s", hipGetErrorString(e));
   *loss = hnp ? hacc / (double)hnp : 0.0;
   if (pairs) *pairs = (int64_t)hnp;
+  return W2V_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// initialisation, sampling tables, synthetic corpora
+// ---------------------------------------------------------------------------------
+namespace w2v {
+bool synth_make_tables(int32_t V, int32_t C, double s, double stay, double in_cluster,
+                       int32_t len_median, double len_sigma, int32_t len_max, uint64_t seed,
+                       SynthTables& t);
+int32_t synth_cluster_of(uint64_t seed, int32_t clusters, int32_t word);
+hipError_t launch_synth_lengths(int64_t n, int64_t first, uint64_t mixed_seed, const uint64_t* lthr,
+                                int32_t len_max, int64_t* len, hipStream_t st);
+hipError_t scan_lengths(const int64_t* len, int64_t* off, int64_t n, void* temp, size_t* temp_bytes,
+                        hipStream_t st);
+hipError_t launch_synth_tokens(int64_t n, int64_t first, uint64_t mixed_seed, int32_t V, int32_t C,
+                               const uint64_t* gthr, const int32_t* members, const int64_t* cstart,
+                               const uint64_t* cthr, uint64_t stay_thr, uint64_t in_thr,
+                               const int64_t* off, int32_t* out, hipStream_t st);
+hipError_t launch_count(const int32_t* tok, int64_t n, unsigned long long* cn, int num_cu,
+                        hipStream_t st);
+void lcg_jump_tables(uint64_t* a, uint64_t* c);
+hipError_t launch_init_syn0(float* syn0, uint16_t* bsyn0, int64_t V, int32_t d, int32_t P,
+                            const uint64_t* ja, const uint64_t* jc, hipStream_t st);
+}  // namespace w2v
+
+extern "C" {
+
+int w2v_init_model(w2v_ctx* c) {
+  if (!c) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  uint64_t ja[64], jc[64];
+  lcg_jump_tables(ja, jc);
+  uint64_t* dj = nullptr;
+  if (hipMalloc(&dj, sizeof(ja) + sizeof(jc)) != hipSuccess) return fail(c, W2V_ERR_NOMEM, "jump table");
+  hipError_t e = hipMemcpy(dj, ja, sizeof(ja), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dj + 64, jc, sizeof(jc), hipMemcpyHostToDevice);
+  const bool bf = c->p.storage == W2V_STORAGE_BF16;
+  const size_t rows = (size_t)c->p.vocab_size * c->P * (bf ? 2 : 4);
+  if (e == hipSuccess) e = hipMemset(bf ? (void*)c->bsyn0 : (void*)c->syn0, 0, rows);
+  if (e == hipSuccess) e = hipMemset(bf ? (void*)c->bsyn1 : (void*)c->syn1, 0, rows);
+  if (e == hipSuccess)
+    e = launch_init_syn0(c->syn0, c->bsyn0, c->p.vocab_size, c->p.layer_size, c->P, dj, dj + 64,
+                         c->ss[0].st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->ss[0].st);
+  (void)hipFree(dj);
+  if (e != hipSuccess) return fail(c, W2V_ERR_HIP, "model initialisation failed: %s", hipGetErrorString(e));
+  return W2V_OK;
+}
+
+int w2v_unigram_table(const uint64_t* cn, int32_t V, int32_t* table, int64_t n, int32_t corrected) {
+  if (!cn || !table || V < 1 || n < 1) return W2V_ERR_ARG;
+  // improved_driver_functions.cpp:45-58 (FP64 threshold walk over the table)
+  double pow_sum = 0.0;
+  for (int32_t i = 0; i < V; i++) pow_sum += pow((double)cn[i], 0.75);
+  int32_t i = corrected ? 0 : 1;
+  double d1 = pow((double)cn[0], 0.75) / pow_sum;
+  if (i >= V) i = V - 1;
+  for (int64_t a = 0; a < n; a++) {
+    table[a] = i;
+    if (a / (double)n > d1) {
+      if (i < V - 1) i++;
+      d1 += pow((double)cn[i], 0.75) / pow_sum;
+    }
+  }
+  return W2V_OK;
+}
+
+int w2v_subsample_probs(const uint64_t* cn, int32_t V, float sample, uint64_t train_words, float* p) {
+  if (!cn || !p || V < 1) return W2V_ERR_ARG;
+  const float base = sample * (float)train_words;  // improved_driver_functions.cpp:71-77
+  for (int32_t i = 0; i < V; i++) {
+    const float ratio = base / (float)cn[i];
+    p[i] = sqrtf(ratio) + ratio;
+  }
+  return W2V_OK;
+}
+
+int32_t w2v_synth_cluster_of(uint64_t seed, int32_t clusters, int32_t word) {
+  if (clusters < 1) return -1;
+  return synth_cluster_of(seed, clusters, word);
+}
+
+int w2v_synth_corpus(w2v_ctx* c, const w2v_synth* cfg, int64_t first, int64_t n) {
+  if (!c || !cfg || first < 0 || n < 0) return W2V_ERR_ARG;
+  if (c->p.vocab_size > INT32_MAX) return fail(c, W2V_ERR_UNSUPPORTED, "vocabulary too large");
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  SynthTables t;
+  if (!synth_make_tables((int32_t)c->p.vocab_size, cfg->clusters, cfg->zipf_s, cfg->stay,
+                         cfg->in_cluster, cfg->len_median, cfg->len_sigma, cfg->len_max, cfg->seed, t))
+    return fail(c, W2V_ERR_ARG, "invalid synthetic corpus configuration");
+  if (n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "at most 2^31-2 sentences");
+  hipStream_t st = c->ss[0].st;
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  if (c->d_corpus) HIP_TRY(c, hipFree(c->d_corpus));
+  if (c->d_off) HIP_TRY(c, hipFree(c->d_off));
+  c->d_corpus = nullptr;
+  c->d_off = nullptr;
+  c->n_tokens = c->n_sent = 0;
+  c->h_off.clear();
+  const uint64_t ms = mix64(cfg->seed);
+  DevBuf tab, len, tmp;
+  auto cleanup = [&]() {
+    if (tab.p) (void)hipFree(tab.p);
+    if (len.p) (void)hipFree(len.p);
+    if (tmp.p) (void)hipFree(tmp.p);
+  };
+  const int32_t V = (int32_t)c->p.vocab_size, C = cfg->clusters;
+  const size_t b_g = (size_t)V * 8, b_m = (size_t)(V - 1) * 4, b_cs = (size_t)(C + 1) * 8,
+               b_ct = (size_t)(V - 1) * 8, b_l = t.lthr.size() * 8;
+  const size_t o_m = b_g, o_cs = (o_m + b_m + 7) / 8 * 8, o_ct = o_cs + b_cs, o_l = o_ct + b_ct;
+  if ((rc = ensure_dev(c, tab, o_l + b_l)) || (rc = ensure_dev(c, len, (size_t)(n + 1) * 8))) {
+    cleanup();
+    return rc;
+  }
+  char* base = (char*)tab.p;
+  hipError_t e = hipMemcpy(base, t.gthr.data(), b_g, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + o_m, t.members.data(), b_m, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + o_cs, t.cstart.data(), b_cs, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + o_ct, t.cthr.data(), b_ct, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + o_l, t.lthr.data(), b_l, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(len.p, 0, (size_t)(n + 1) * 8);
+  if (e == hipSuccess)
+    e = launch_synth_lengths(n, first, ms, (const uint64_t*)(base + o_l), cfg->len_max, (int64_t*)len.p, st);
+  if (e == hipSuccess && hipMalloc(&c->d_off, (size_t)(n + 1) * 8) != hipSuccess) e = hipErrorOutOfMemory;
+  size_t tb = 0;
+  if (e == hipSuccess) e = scan_lengths((const int64_t*)len.p, c->d_off, n, nullptr, &tb, st);
+  if (e == hipSuccess) {
+    rc = ensure_dev(c, tmp, tb);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+    e = scan_lengths((const int64_t*)len.p, c->d_off, n, tmp.p, &tb, st);
+  }
+  if (e == hipSuccess) {
+    c->h_off.resize(n + 1);
+    e = hipMemcpyAsync(c->h_off.data(), c->d_off, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) {
+    const int64_t nt = c->h_off[n];
+    if (hipMalloc(&c->d_corpus, (size_t)(nt > 0 ? nt : 1) * 4) != hipSuccess) e = hipErrorOutOfMemory;
+    else
+      e = launch_synth_tokens(n, first, ms, V, C, (const uint64_t*)base, (const int32_t*)(base + o_m),
+                              (const int64_t*)(base + o_cs), (const uint64_t*)(base + o_ct), t.stay_thr,
+                              t.in_thr, c->d_off, c->d_corpus, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) {
+      c->n_tokens = nt;
+      c->n_sent = n;
+    }
+  }
+  cleanup();
+  if (e != hipSuccess) {
+    c->h_off.clear();
+    return fail(c, W2V_ERR_HIP, "synthetic corpus generation failed: %s", hipGetErrorString(e));
+  }
+  return W2V_OK;
+}
+
+int w2v_corpus_info(w2v_ctx* c, int64_t* n_tokens, int64_t* n_sent) {
+  if (!c) return W2V_ERR_ARG;
+  if (n_tokens) *n_tokens = c->n_tokens;
+  if (n_sent) *n_sent = c->n_sent;
+  return W2V_OK;
+}
+
+int w2v_corpus_download(w2v_ctx* c, int64_t sb, int64_t se, int32_t* tokens, int64_t* offsets) {
+  if (!c || !offsets || sb < 0 || se > c->n_sent || sb > se) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  const int64_t t0 = c->h_off[sb], t1 = c->h_off[se];
+  for (int64_t s = sb; s <= se; s++) offsets[s - sb] = c->h_off[s] - t0;
+  if (t1 > t0 && tokens)
+    HIP_TRY(c, hipMemcpy(tokens, c->d_corpus + t0, (size_t)(t1 - t0) * 4, hipMemcpyDeviceToHost));
+  return W2V_OK;
+}
+
+int w2v_corpus_counts(w2v_ctx* c, uint64_t* counts) {
+  if (!c || !counts) return W2V_ERR_ARG;
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  const size_t b = (size_t)c->p.vocab_size * 8;
+  unsigned long long* dc = nullptr;
+  if (hipMalloc(&dc, b) != hipSuccess) return fail(c, W2V_ERR_NOMEM, "count buffer");
+  hipError_t e = hipMemset(dc, 0, b);
+  if (e == hipSuccess && c->n_tokens > 0) e = launch_count(c->d_corpus, c->n_tokens, dc, c->num_cu, c->ss[0].st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->ss[0].st);
+  if (e == hipSuccess) e = hipMemcpy(counts, dc, b, hipMemcpyDeviceToHost);
+  (void)hipFree(dc);
+  if (e != hipSuccess) return fail(c, W2V_ERR_HIP, "corpus counts failed: %s", hipGetErrorString(e));
   return W2V_OK;
 }
 

@@ -91,8 +91,13 @@ __device__ __forceinline__ double neg_log_sigmoid(double x) {
   return x > 0.0 ? log1p(exp(-x)) : -x + log1p(exp(x));
 }
 
+__device__ __forceinline__ float row_at(const float* f, const uint16_t* b, size_t i) {
+  return b ? __uint_as_float((uint32_t)b[i] << 16) : f[i];
+}
+
 __global__ void __launch_bounds__(256) heldout_loss_kernel(
-    const float* __restrict__ syn0, const float* __restrict__ syn1, int P, int W, int K,
+    const float* __restrict__ syn0, const float* __restrict__ syn1, const uint16_t* __restrict__ bsyn0,
+    const uint16_t* __restrict__ bsyn1, int P, int W, int K,
     const int32_t* __restrict__ tokens, const int64_t* __restrict__ off, int64_t n_sent,
     const int32_t* __restrict__ unigram, int64_t n_unigram, uint64_t mixed_seed,
     double* __restrict__ acc, unsigned long long* __restrict__ pairs) {
@@ -113,13 +118,14 @@ __global__ void __launch_bounds__(256) heldout_loss_kernel(
       }
       for (int c = w - W; c <= w + W; c++) {
         if (c < 0 || c >= L || c == w) continue;
-        const float* u = syn0 + (size_t)tokens[b + c] * P;
+        const size_t ub = (size_t)tokens[b + c] * P;
         double l = 0.0;
         for (int k = 0; k <= K; k++) {
           const int t = k == 0 ? cw : readlane_i(negv, k - 1);
-          const float* v = syn1 + (size_t)t * P;
+          const size_t vb = (size_t)t * P;
           float part = 0.0f;
-          for (int j = lane; j < P; j += 64) part = fmaf(u[j], v[j], part);
+          for (int j = lane; j < P; j += 64)
+            part = fmaf(row_at(syn0, bsyn0, ub + j), row_at(syn1, bsyn1, vb + j), part);
           const float f = wave_sum(part);
           l += k == 0 ? neg_log_sigmoid((double)f) : neg_log_sigmoid(-(double)f);
         }
@@ -134,15 +140,16 @@ __global__ void __launch_bounds__(256) heldout_loss_kernel(
   }
 }
 
-hipError_t launch_heldout_loss(const float* syn0, const float* syn1, int P, int W, int K,
+hipError_t launch_heldout_loss(const float* syn0, const float* syn1, const uint16_t* bsyn0,
+                               const uint16_t* bsyn1, int P, int W, int K,
                                const int32_t* tokens, const int64_t* off, int64_t n_sent,
                                const int32_t* unigram, int64_t n_unigram, uint64_t mixed_seed,
                                double* acc, unsigned long long* pairs, int num_cu, hipStream_t st) {
   int64_t blocks = (n_sent + 3) / 4;
   if (blocks > (int64_t)num_cu * 8) blocks = (int64_t)num_cu * 8;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(heldout_loss_kernel, dim3((unsigned)blocks), dim3(256), 0, st, syn0, syn1, P,
-                     W, K, tokens, off, n_sent, unigram, n_unigram, mixed_seed, acc, pairs);
+  hipLaunchKernelGGL(heldout_loss_kernel, dim3((unsigned)blocks), dim3(256), 0, st, syn0, syn1, bsyn0,
+                     bsyn1, P, W, K, tokens, off, n_sent, unigram, n_unigram, mixed_seed, acc, pairs);
   return hipGetLastError();
 }
 

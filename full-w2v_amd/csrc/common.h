@@ -79,9 +79,11 @@ __host__ __device__ __forceinline__ uint64_t pos_key(int32_t epoch, int64_t pos)
 // Kernel argument block of the sentence-update kernels.
 // ---------------------------------------------------------------------------------
 struct UpdateArgs {
-  // model (device layout: row r at r*P, pad columns d..P-1 kept at zero)
+  // model (device layout: row r at r*P, pad columns d..P-1 kept at zero); FP32 or bf16
   float* syn0;
   float* syn1;
+  uint16_t* bsyn0;
+  uint16_t* bsyn1;
   // flush targets: the model in normal mode, FP32 delta buffers in delta mode
   float* out0;
   float* out1;

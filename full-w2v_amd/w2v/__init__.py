@@ -27,7 +27,9 @@ EXPORTS = [
     "w2v_sigmoid_tables", "w2v_submit_padded", "w2v_submit_csr", "w2v_sync", "w2v_sync_all",
     "w2v_set_delta_mode", "w2v_delta_download", "w2v_tables_upload", "w2v_set_schedule",
     "w2v_corpus_upload", "w2v_train_device", "w2v_get_stats", "w2v_heldout_loss",
-    "w2v_kernel_time",
+    "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
+    "w2v_synth_corpus", "w2v_synth_cluster_of", "w2v_corpus_info", "w2v_corpus_download",
+    "w2v_corpus_counts",
 ]
 
 
@@ -40,6 +42,14 @@ class Params(ctypes.Structure):
         ("device", ctypes.c_int32), ("streams", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
     ]
+
+
+class Synth(ctypes.Structure):
+    _fields_ = [("clusters", ctypes.c_int32), ("len_median", ctypes.c_int32),
+                ("len_max", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("zipf_s", ctypes.c_double), ("stay", ctypes.c_double),
+                ("in_cluster", ctypes.c_double), ("len_sigma", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
 
 
 class Stats(ctypes.Structure):
@@ -78,6 +88,14 @@ def lib() -> ctypes.CDLL:
             "w2v_heldout_loss": [vp, vp, vp, i64, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_int64)],
             "w2v_kernel_time": [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)],
+            "w2v_init_model": [vp],
+            "w2v_unigram_table": [vp, i32, vp, i64, i32],
+            "w2v_subsample_probs": [vp, i32, ctypes.c_float, u64, vp],
+            "w2v_synth_corpus": [vp, ctypes.POINTER(Synth), i64, i64],
+            "w2v_synth_cluster_of": [u64, i32, i32],
+            "w2v_corpus_info": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
+            "w2v_corpus_download": [vp, i64, i64, vp, vp],
+            "w2v_corpus_counts": [vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -97,6 +115,35 @@ class W2VError(RuntimeError):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def unigram_table(counts, n=100_000_000, corrected=False):
+    """Negative-sampling table built by the library's host code."""
+    c = np.ascontiguousarray(counts, np.uint64)
+    t = np.zeros(int(n), np.int32)
+    rc = lib().w2v_unigram_table(_ptr(c), len(c), _ptr(t), int(n), 1 if corrected else 0)
+    if rc:
+        raise W2VError(rc, "w2v_unigram_table")
+    return t
+
+
+def subsample_probs(counts, sample, train_words):
+    c = np.ascontiguousarray(counts, np.uint64)
+    p = np.zeros(len(c), np.float32)
+    rc = lib().w2v_subsample_probs(_ptr(c), len(c), float(sample), int(train_words), _ptr(p))
+    if rc:
+        raise W2VError(rc, "w2v_subsample_probs")
+    return p
+
+
+def synth_cluster_of(seed, clusters, words):
+    f = lib().w2v_synth_cluster_of
+    return np.array([f(seed, clusters, int(w)) for w in words], np.int32)
+
+
+def synth_config(clusters=64, len_median=22, len_max=1000, zipf_s=1.0, stay=0.9, in_cluster=0.8,
+                 len_sigma=1.0, seed=1):
+    return Synth(clusters, len_median, len_max, 0, zipf_s, stay, in_cluster, len_sigma, seed)
 
 
 def sigmoid_tables():
@@ -206,6 +253,29 @@ class Trainer:
         self._chk(lib().w2v_heldout_loss(self.h, _ptr(t), _ptr(o), len(o) - 1, ctypes.byref(loss),
                                          ctypes.byref(pairs)))
         return loss.value, pairs.value
+
+    def init_model(self):
+        self._chk(lib().w2v_init_model(self.h))
+
+    def synth_corpus(self, cfg, n_sentences, first=0):
+        self._chk(lib().w2v_synth_corpus(self.h, ctypes.byref(cfg), int(first), int(n_sentences)))
+
+    def corpus_info(self):
+        nt, ns = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(lib().w2v_corpus_info(self.h, ctypes.byref(nt), ctypes.byref(ns)))
+        return nt.value, ns.value
+
+    def corpus_download(self, sent_begin, sent_end):
+        off = np.zeros(sent_end - sent_begin + 1, np.int64)
+        self._chk(lib().w2v_corpus_download(self.h, sent_begin, sent_end, None, _ptr(off)))
+        tok = np.zeros(int(off[-1]), np.int32)
+        self._chk(lib().w2v_corpus_download(self.h, sent_begin, sent_end, _ptr(tok), _ptr(off)))
+        return tok, off
+
+    def corpus_counts(self):
+        c = np.zeros(self.V, np.uint64)
+        self._chk(lib().w2v_corpus_counts(self.h, _ptr(c)))
+        return c
 
     def kernel_time(self, stream=0):
         ms = ctypes.c_double()

@@ -144,6 +144,50 @@ int w2v_heldout_loss(w2v_ctx* ctx, const int32_t* tokens, const int64_t* offsets
  * events around each launch) and their number. */
 int w2v_kernel_time(w2v_ctx* ctx, int32_t stream, double* ms, uint64_t* launches);
 
+/* ---- model initialisation and sampling tables (host side of InitNet) ----------- */
+/* InitNet's initialisation (improved_driver_functions.cpp:24-31) done on the device:
+ * syn0[i] = (((x_i & 0xFFFF)/65536 - 0.5)/d) over the LCG stream from seed 1, row-major
+ * over V x d; syn1neg = 0. Bit-identical to the sequential host loop. */
+int w2v_init_model(w2v_ctx* ctx);
+/* Unigram^0.75 negative-sampling table (improved_driver_functions.cpp:45-58). With
+ * corrected = 0 it keeps the reference's label shift (label 1 receives p_0's share);
+ * corrected = 1 gives word i exactly p_i. Host only, no context needed. */
+int w2v_unigram_table(const uint64_t* counts, int32_t vocab_size, int32_t* table, int64_t n,
+                      int32_t corrected);
+/* Subsampling keep probabilities sqrt(r)+r, r = sample*train_words/count
+ * (improved_driver_functions.cpp:71-77). Host only. */
+int w2v_subsample_probs(const uint64_t* counts, int32_t vocab_size, float sample,
+                        uint64_t train_words, float* p);
+
+/* ---- synthetic corpora (no counterpart in the reference, which downloads its
+ * datasets: data/getText8.sh, data/getBillion.sh) ------------------------------- */
+typedef struct w2v_synth {
+  int32_t clusters;   /* words 1..V-1 are assigned uniformly to this many clusters */
+  int32_t len_median; /* log-normal sentence-length median; 0 = every sentence len_max */
+  int32_t len_max;    /* sentence lengths are clamped to [1, len_max] */
+  int32_t reserved;
+  double zipf_s;      /* Zipf exponent of the global and per-cluster distributions */
+  double stay;        /* probability the cluster Markov chain stays in its cluster */
+  double in_cluster;  /* probability a token is drawn from its cluster's Zipf */
+  double len_sigma;   /* log-normal sigma */
+  uint64_t seed;
+} w2v_synth;
+/* Generates sentences [first_sentence, first_sentence + n_sentences) of the synthetic
+ * cluster-Markov Zipf corpus (vocabulary = the context's V; id 0 is </s> and never
+ * emitted) directly into the device corpus, replacing it. */
+int w2v_synth_corpus(w2v_ctx* ctx, const w2v_synth* cfg, int64_t first_sentence,
+                     int64_t n_sentences);
+/* Cluster label of a word of the synthetic corpus (planted-cluster evaluation). */
+int32_t w2v_synth_cluster_of(uint64_t seed, int32_t clusters, int32_t word);
+int w2v_corpus_info(w2v_ctx* ctx, int64_t* n_tokens, int64_t* n_sentences);
+/* Copies sentences [sent_begin, sent_end) of the device corpus to the host; offsets
+ * (sent_end - sent_begin + 1 entries) are relative to the first copied token. With
+ * tokens == NULL only the offsets are written (to size the token buffer). */
+int w2v_corpus_download(w2v_ctx* ctx, int64_t sent_begin, int64_t sent_end, int32_t* tokens,
+                        int64_t* offsets);
+/* Occurrence count of every word id in the device corpus (counts has V entries). */
+int w2v_corpus_counts(w2v_ctx* ctx, uint64_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

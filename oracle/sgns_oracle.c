@@ -208,8 +208,8 @@ void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int
           double f = 0.0;
           for (int32_t j = 0; j < d; j++)
             f += (bf16_emul ? bf16_round(u[j]) * bf16_round(t[j]) : u[j] * t[j]);
-          double g = sig_grad(bf16_emul ? (double)(float)f : f, k == 0, alpha, sig_mode);
-          G[(size_t)nc * (K + 1) + k] = bf16_emul ? bf16_round(g) : g;
+          double g = sig_grad(f, k == 0, alpha, sig_mode);
+          G[(size_t)nc * (K + 1) + k] = g;
         }
         nc++;
       }
@@ -220,10 +220,7 @@ void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int
         double* du = dU + (size_t)nc * d;
         for (int32_t j = 0; j < d; j++) {
           double acc = 0.0;
-          for (int32_t k = 0; k <= K; k++) {
-            double tj = T[(size_t)k * d + j];
-            acc += G[(size_t)nc * (K + 1) + k] * (bf16_emul ? bf16_round(tj) : tj);
-          }
+          for (int32_t k = 0; k <= K; k++) acc += G[(size_t)nc * (K + 1) + k] * T[(size_t)k * d + j];
           du[j] = acc;
         }
         nc++;
@@ -235,8 +232,7 @@ void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int
           int32_t q = 0;
           for (int32_t c = lo; c <= hi; c++) {
             if (c == w) continue;
-            double uj = U[(size_t)c * d + j];
-            acc += G[(size_t)q * (K + 1) + k] * (bf16_emul ? bf16_round(uj) : uj);
+            acc += G[(size_t)q * (K + 1) + k] * U[(size_t)c * d + j];
             q++;
           }
           syn1[(size_t)tgt[k] * d + j] += acc;

@@ -49,8 +49,9 @@ float orc_learning_rate(float alpha0, uint64_t word_count_actual, int32_t epochs
  * sig_mode: 0 = reference table sigmoid, 1 = exact sigmoid.
  * tile:     0 = sequential pair updates (reference order),
  *           1 = window tile (all (context,target) dots of one centre from one snapshot).
- * bf16_emul (tile only): round the tile-product operands to bfloat16 (RNE) as the
- *           MFMA kernel does. */
+ * bf16_emul (tile only): round the operands of the dot-product tile S = C T^T to
+ *           bfloat16 (round to nearest even), as the MFMA kernel feeds them; the
+ *           sigmoid, G and both updates stay unrounded. */
 void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
                            const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
                            int sig_mode, int tile, int bf16_emul);

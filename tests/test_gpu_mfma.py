@@ -1,0 +1,121 @@
+"""GPU parity of the BF16-MFMA window-tile kernel (deterministic mode, delta buffers).
+
+Reference: the FP64 oracle in tile mode with the dot-product operands rounded to bf16
+(what the MFMA kernel feeds the matrix cores), starting from the bf16-stored model.
+Tolerance: row-wise relative 1e-4 (FP32 accumulation on the GPU vs FP64; rare bf16
+rounding-boundary straddles of the working context rows). The error against the FP64
+sequential oracle (the reference method) is measured and printed for each case.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_plain import model, row_rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_TILE = 1e-4
+
+
+def run_mfma(w2v, V, d, W, K, syn0, syn1, tokens, off, neg, alpha, sigmoid="exact"):
+    tr = w2v.Trainer(V, d, W, K, msl=max(1, int(np.diff(off).max())), storage="bf16",
+                     kernel="mfma", sigmoid=sigmoid, mode="deterministic")
+    tr.upload(syn0, syn1)
+    b0, b1 = tr.download()  # the bf16-stored starting point
+    tr.set_delta_mode(True)
+    tr.submit_csr(tokens, off, neg if K else None, alpha)
+    g0, g1 = tr.delta_download()
+    tr.close()
+    return b0, b1, g0, g1
+
+
+def oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile, bf16):
+    s0, s1 = b0.astype(np.float64), b1.astype(np.float64)
+    a0, a1 = s0.copy(), s1.copy()
+    for s in range(len(off) - 1):
+        t = tokens[off[s]:off[s + 1]]
+        n = neg[off[s] * K:off[s + 1] * K] if K else None
+        a0, a1 = orc.sgns_sentence(a0, a1, W, K, t, n, float(np.float32(alpha[s])), sig_mode=1,
+                                   tile=tile, bf16=bf16)
+    return a0 - s0, a1 - s1
+
+
+CASES = [
+    (40, 64, 2, 3, [7]),
+    (40, 128, 5, 5, [1]),
+    (40, 128, 5, 5, [11]),
+    (60, 300, 5, 5, [22]),
+    (50, 100, 3, 2, [30, 4]),
+    (200, 128, 5, 5, [130]),
+    (30, 64, 7, 15, [40]),
+    (30, 256, 1, 0, [9]),
+]
+
+
+@pytest.mark.parametrize("V,d,W,K,lens", CASES)
+def test_mfma_tile_matches_oracle(orc, w2vmod, V, d, W, K, lens):
+    rng = np.random.default_rng(V + d + W * 7 + K * 13 + len(lens))
+    syn0, syn1 = model(V, d, rng)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(0, V, off[-1]).astype(np.int32)
+    neg = rng.integers(0, V, off[-1] * K).astype(np.int32)
+    alpha = np.full(len(lens), 0.025, np.float32)
+    b0, b1, g0, g1 = run_mfma(w2vmod, V, d, W, K, syn0, syn1, tokens, off, neg, alpha)
+    t0, t1 = oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile=1, bf16=1)
+    q0, q1 = oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile=0, bf16=0)
+    et = max(row_rel_err(g0, t0), row_rel_err(g1, t1))
+    eq = max(float(np.abs(g0 - q0).max() / np.abs(q0).max()), float(np.abs(g1 - q1).max() / np.abs(q1).max()))
+    print(f"mfma V={V} d={d} W={W} K={K} lens={lens}: vs tile+bf16 oracle {et:.2e}, "
+          f"vs FP64 sequential {eq:.2e}", flush=True)
+    assert et <= TOL_TILE
+
+
+def test_mfma_hogwild_and_stochastic_rounding(orc, w2vmod):
+    """Hogwild run: finite, moves the model; stochastic rounding keeps the mean of many
+    tiny updates (a round-to-nearest add of a delta below half an ulp would drop them)."""
+    V, d, W, K = 400, 128, 5, 5
+    rng = np.random.default_rng(2)
+    syn0 = orc.init_syn0(V, d)
+    syn1 = np.zeros((V, d), np.float32)
+    lens = rng.integers(1, 60, 3000)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(0, V, off[-1]).astype(np.int32)
+    neg = rng.integers(0, V, off[-1] * K).astype(np.int32)
+    alpha = np.full(len(lens), 0.025, np.float32)
+    tr = w2vmod.Trainer(V, d, W, K, msl=64, storage="bf16", kernel="mfma")
+    tr.upload(syn0, syn1)
+    tr.submit_csr(tokens, off, neg, alpha)
+    s0, s1 = tr.download()
+    st = tr.stats()
+    tr.close()
+    assert np.isfinite(s0).all() and np.isfinite(s1).all()
+    assert st["words_trained"] == off[-1]
+    assert np.abs(s1).max() > 0
+
+
+def test_device_corpus_matches_oracle_generator(orc, w2vmod):
+    V, n = 3001, 2000
+    tr = w2vmod.Trainer(V, 64, 2, 2)
+    cfg = w2vmod.synth_config(clusters=16, len_median=22, len_max=300, seed=5)
+    tr.synth_corpus(cfg, n)
+    tok, off = tr.corpus_download(0, n)
+    counts = tr.corpus_counts()
+    tr.close()
+    ocfg = orc.synth_cfg(V=V, clusters=16, len_median=22, len_max=300, seed=5)
+    otok, ooff = orc.synth_corpus(ocfg, n)
+    assert np.array_equal(off, ooff) and np.array_equal(tok, otok)
+    assert np.array_equal(counts, np.bincount(tok, minlength=V).astype(np.uint64))
+
+
+@pytest.mark.parametrize("storage,kernel", [("fp32", "plain"), ("bf16", "mfma")])
+def test_device_init_matches_initnet(orc, w2vmod, storage, kernel):
+    V, d = 777, 300
+    tr = w2vmod.Trainer(V, d, 2, 2, storage=storage, kernel=kernel)
+    tr.init_model()
+    s0, s1 = tr.download()
+    tr.close()
+    ref = orc.init_syn0(V, d)
+    if storage == "bf16":
+        u = ref.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        ref = u.astype(np.uint32).view(np.float32)
+    assert np.array_equal(s0, ref) and not s1.any()

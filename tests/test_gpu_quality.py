@@ -1,0 +1,92 @@
+"""Statistical parity in Hogwild mode: held-out SGNS loss and planted-cluster recovery.
+
+The same synthetic cluster-Markov Zipf corpus is trained for one epoch by
+  * the CPU reference trainer (oracle, FP32 Hogwild, the reference's host sampling),
+  * the FP32 plain GPU kernel (device sampling),
+  * the BF16-MFMA GPU kernel (device sampling, stochastic rounding).
+Targets: held-out loss within 1% of the CPU trainer and nearest-neighbour cluster
+recovery within 1 point (0.01).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+V, D, W, K = 20001, 64, 5, 5
+N_SENT = 120_000
+SEED = 1
+
+
+def nn_recovery(syn0, labels, words, k=10):
+    x = syn0[words].astype(np.float64)
+    x /= np.linalg.norm(x, axis=1, keepdims=True) + 1e-12
+    sim = x @ x.T
+    np.fill_diagonal(sim, -np.inf)
+    nn = np.argpartition(-sim, k, axis=1)[:, :k]
+    lab = labels[words]
+    return float(np.mean(lab[nn] == lab[:, None]))
+
+
+@pytest.fixture(scope="module")
+def runs(orc, w2vmod):
+    cfg = w2vmod.synth_config(clusters=64, len_median=22, len_max=1000, seed=SEED)
+    tr = w2vmod.Trainer(V, D, W, K)
+    tr.synth_corpus(cfg, N_SENT)
+    tok, off = tr.corpus_download(0, N_SENT)
+    cn = tr.corpus_counts()
+    hcfg = w2vmod.synth_config(clusters=64, len_median=22, len_max=1000, seed=2)
+    tr.synth_corpus(hcfg, 4000)
+    htok, hoff = tr.corpus_download(0, 4000)
+    tr.close()
+    cn[0] = N_SENT
+    cn = np.maximum(cn, 1)
+    T = int(off[-1])
+    uni = w2vmod.unigram_table(cn, 10_000_000)
+    subs = w2vmod.subsample_probs(cn, 1e-4, T)
+    labels = w2vmod.synth_cluster_of(SEED, 64, np.arange(V))
+    probe = np.arange(2, 1002)
+    out = {}
+    # CPU reference trainer: corpus lines terminated by </s> (-cache-eos layout)
+    lens = np.diff(off)
+    starts = off[:-1] + np.arange(N_SENT)
+    stream = np.zeros(T + N_SENT + 1, np.int32)
+    for s in range(N_SENT):
+        stream[starts[s]:starts[s] + lens[s]] = tok[off[s]:off[s + 1]]
+    s0 = orc.init_syn0(V, D)
+    s1 = np.zeros((V, D), np.float32)
+    threads = min(16, len(os.sched_getaffinity(0)))
+    orc.train_hogwild(stream, starts.astype(np.int64), T, uni, subs, s0, s1, D, W, K, 1000, 1,
+                      threads, 0.025, 1e-4)
+    out["cpu"] = (orc.heldout_loss(s0, s1, W, K, htok, hoff, uni, 77)[0], nn_recovery(s0, labels, probe))
+    for kernel, storage in (("plain", "fp32"), ("mfma", "bf16")):
+        g = w2vmod.Trainer(V, D, W, K, storage=storage, kernel=kernel, seed=SEED)
+        g.corpus_upload(tok, off)
+        g.tables_upload(uni, subs)
+        g.set_schedule(0.025, 1, T)
+        g.init_model()
+        g.train_device(0, 0, N_SENT)
+        g0, g1 = g.download()
+        g.close()
+        out[kernel] = (orc.heldout_loss(g0, g1, W, K, htok, hoff, uni, 77)[0],
+                       nn_recovery(g0, labels, probe))
+    print("held-out loss / NN recovery:", out, flush=True)
+    return out
+
+
+def test_plain_heldout_loss_within_1pct_of_cpu(runs):
+    assert abs(runs["plain"][0] - runs["cpu"][0]) <= 0.01 * runs["cpu"][0]
+
+
+def test_mfma_heldout_loss_within_1pct_of_cpu(runs):
+    assert abs(runs["mfma"][0] - runs["cpu"][0]) <= 0.01 * runs["cpu"][0]
+
+
+def test_mfma_heldout_loss_within_1pct_of_plain(runs):
+    assert abs(runs["mfma"][0] - runs["plain"][0]) <= 0.01 * runs["plain"][0]
+
+
+def test_nn_recovery_within_1_point(runs):
+    assert abs(runs["plain"][1] - runs["cpu"][1]) <= 0.01
+    assert abs(runs["mfma"][1] - runs["cpu"][1]) <= 0.01
