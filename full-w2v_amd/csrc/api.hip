@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -98,7 +99,7 @@ struct w2v_ctx {
   int32_t epochs = 1;
   uint64_t train_words = 0;
   uint64_t words_read = 0;
-  uint64_t launches = 0;
+  std::atomic<uint64_t> launches{0};
   std::string err;
 };
 
@@ -223,23 +224,6 @@ int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a) {
   a.queue = s.queue;
   HIP_TRY(c, launch_zero_u32(s.queue, 1, s.st));
   const int dm = c->delta_mode ? 1 : 0;
-  if (!c->cfg_blocks[dm]) {
-    // waves per block and persistent grid from the per-wave LDS ring and occupancy
-    const bool mf = c->p.kernel == W2V_KERNEL_MFMA;
-    const size_t lw = mf ? mfma_lds_per_wave(c->p.window, c->p.negative, c->P, dm)
-                         : plain_lds_per_wave(c->p.window, c->P);
-    const size_t lmax = 160 * 1024;
-    if (lw > lmax)
-      return fail(c, W2V_ERR_UNSUPPORTED, "per-wave LDS of %zu bytes exceeds 160 KiB", lw);
-    int wpb = 4;
-    while (wpb > 1 && lw * wpb > lmax) wpb--;
-    int bpc = 0;
-    hipError_t oe = mf ? occupancy_mfma(c->P, c->p.negative, wpb, lw * wpb, &bpc)
-                       : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
-    if (oe != hipSuccess || bpc < 1) bpc = 1;
-    c->cfg_wpb[dm] = wpb;
-    c->cfg_blocks[dm] = bpc * c->num_cu;
-  }
   int wpb = c->cfg_wpb[dm], blocks = c->cfg_blocks[dm];
   if (c->p.mode == W2V_MODE_DETERMINISTIC) {
     wpb = 1;
@@ -339,6 +323,24 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
       hipMemcpy(c->d_nexp, ne, sizeof(ne), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->d_counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess)
     return bail(fail(c, W2V_ERR_HIP, "table upload failed"));
+  // launch shape of the update kernel, for normal and delta mode: waves per block from
+  // the per-wave LDS ring, persistent grid from the occupancy at that LDS size
+  for (int dm = 0; dm < 2; dm++) {
+    const bool mf = p.kernel == W2V_KERNEL_MFMA;
+    const size_t lw = mf ? mfma_lds_per_wave(p.window, p.negative, c->P, dm != 0)
+                         : plain_lds_per_wave(p.window, c->P);
+    const size_t lmax = 160 * 1024;
+    if (lw > lmax)
+      return bail(fail(c, W2V_ERR_UNSUPPORTED, "per-wave LDS of %zu bytes exceeds 160 KiB", lw));
+    int wpb = 4;
+    while (wpb > 1 && lw * wpb > lmax) wpb--;
+    int bpc = 0;
+    hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, wpb, lw * wpb, &bpc)
+                       : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
+    if (oe != hipSuccess || bpc < 1) bpc = 1;
+    c->cfg_wpb[dm] = wpb;
+    c->cfg_blocks[dm] = bpc * c->num_cu;
+  }
   c->ss.resize(p.streams);
   for (auto& s : c->ss) {
     if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess)

@@ -1,0 +1,518 @@
+// full-w2v: command-line trainer with the reference's flags and output
+// (FULL-W2V-driver.cu:427-472, improved_driver_functions.cpp:403-535), driving libw2v.
+//
+// Two training paths:
+//  * host sampling (default, the reference's Train_SGNS, FULL-W2V-driver.cu:80-354):
+//    -threads host threads each own a contiguous slice of sentences, draw subsampling
+//    and negatives from their LCG stream (seed = thread id), build GPU sentences of up
+//    to -msl kept tokens and submit batches of -kernel-batch-size sentences round-robin
+//    over -streams library streams;
+//  * device sampling (-device-sampling 1): the corpus is copied to HBM once and each
+//    epoch is trained by w2v_train_device in chunks, with subsampling, negatives and the
+//    learning rate computed on the device.
+#include <pthread.h>
+#include <sched.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "text.h"
+#include "w2v.h"
+
+using namespace w2vhost;
+
+namespace {
+
+constexpr int64_t kUnigramTableSize = 100000000;  // UNIGRAM_TABLE_SIZE (FULL-W2V-driver.cu:61)
+
+struct Options {
+  std::string output, save_vocab, read_vocab, train, save_corpus, save_cache, read_cache;
+  int binary = 0, debug = 2;
+  bool recount = false, do_export = true, cache_eos = false, pre_shuffle = false, epoch_shuffle = false;
+  int negative = 5, min_count = 5, threads = 12, streams = 4, epochs = 5, window = 5,
+      kernel_batch = 200, layer = 100, msl = 1000;
+  float alpha = 0.025f, sample = 1e-3f;
+  // options of this implementation
+  std::string precision = "fp32";
+  int device_sampling = 0, device = 0, exact_sigmoid = 0;
+  unsigned long long seed = 1;
+};
+
+int arg_pos(const char* s, int argc, char** argv) {
+  for (int a = 1; a < argc; a++)
+    if (!strcmp(s, argv[a])) return a;
+  return -1;
+}
+
+void usage(const char* prog) {
+  printf("SGNS word2vec trainer on AMD Instinct MI355X (gfx950)\n\n");
+  printf("Options:\n** FILES & I/O **\n");
+  printf("\t-output <file>\n\t\tUse <file> to save the resulting word vectors\n");
+  printf("\t-no-export\n\t\tSkips exporting vectors to file after training\n");
+  printf("\t-binary <int>\n\t\tSave the resulting vectors in binary mode; default is 0 (off)\n");
+  printf("\t-debug <int>\n\t\tSet the debug mode (default = 2 = more info during training)\n");
+  printf("\t-save-vocab <file>\n\t\tThe vocabulary will be saved to <file>\n");
+  printf("\t-read-vocab <file>\n\t\tThe vocabulary will be read from <file>\n");
+  printf("\t-train <file>\n\t\tUse text data from <file> to train the model\n");
+  printf("\t-min-count <int>\n\t\tDiscard words that appear less than <int> times; default is 5\n");
+  printf("\t-recount\n\t\tReconstruct vocabulary counts from training data\n");
+  printf("\t-save-corpus <file>\n\t-save-corpus-cache <file>\n\t-read-corpus-cache <file>\n");
+  printf("\t-cache-eos\n\t\tInclude cached EOS tokens from corpus paragraph delimiters\n");
+  printf("\t-pre-shuffle\n\t-epoch-shuffle\n");
+  printf("** TRAINING PARAMETERS **\n");
+  printf("\t-size <int>\n\t\tSet size of word vectors; default is 100\n");
+  printf("\t-epoch(s), -iter(s) <int>\n\t\tNumber of training epochs (default 5)\n");
+  printf("\t-alpha <float>\n\t\tSet the starting learning rate; default is 0.025\n");
+  printf("\t-window <int>\n\t\tSet max skip length between words; default is 5\n");
+  printf("\t-sample <float>\n\t\tSubsampling threshold; default is 1e-3\n");
+  printf("\t-negative(s) <int>\n\t\tNumber of negative examples; default is 5\n");
+  printf("** PARALLEL UTILIZATION **\n");
+  printf("\t-thread(s) <int>\n\t\tUse <int> host threads (default 12)\n");
+  printf("\t-kernel-batch-size <int>\n\t\tSentences per kernel launch; default is 200\n");
+  printf("\t-streams <int>\n\t\tGPU streams each host thread manages; default is 4\n");
+  printf("\t-max-sentence-length, -msl <int>\n\t\tThe maximum length of a single sentence\n");
+  printf("** MI355X OPTIONS **\n");
+  printf("\t-precision <fp32|bf16>\n\t\tfp32: per-pair FP32 kernel; bf16: bf16 rows, MFMA window tile\n");
+  printf("\t-device-sampling <0|1>\n\t\tSubsample and draw negatives on the GPU from a resident corpus\n");
+  printf("\t-exact-sigmoid <0|1>\n\t\tUse 1/(1+exp(-x)) instead of the 1000-entry table\n");
+  printf("\t-device <int>\n\t-seed <int>\n");
+  printf("\nExamples:\n./%s -train data.txt -output vec.txt -size 128 -window 5 -sample 1e-4 -negative 5 -epoch 3\n\n",
+         prog);
+}
+
+Options parse(int argc, char** argv) {
+  Options o;
+  if (argc == 1 || arg_pos("-help", argc, argv) > 0) {
+    usage(argv[0]);
+    exit(0);
+  }
+  int i;
+  auto str = [&](const char* f, std::string& dst) {
+    if ((i = arg_pos(f, argc, argv)) > 0 && i + 1 < argc) dst = argv[i + 1];
+  };
+  auto num = [&](const char* f, int& dst) {
+    if ((i = arg_pos(f, argc, argv)) > 0 && i + 1 < argc) dst = atoi(argv[i + 1]);
+  };
+  str("-output", o.output);
+  if (arg_pos("-no-export", argc, argv) > 0) o.do_export = false;
+  num("-binary", o.binary);
+  num("-debug", o.debug);
+  str("-save-vocab", o.save_vocab);
+  str("-read-vocab", o.read_vocab);
+  str("-train", o.train);
+  num("-min-count", o.min_count);
+  if (arg_pos("-recount", argc, argv) > 0) o.recount = true;
+  str("-save-corpus", o.save_corpus);
+  str("-save-corpus-cache", o.save_cache);
+  str("-read-corpus-cache", o.read_cache);
+  if (arg_pos("-cache-eos", argc, argv) > 0) o.cache_eos = true;
+  if (arg_pos("-pre-shuffle", argc, argv) > 0) o.pre_shuffle = true;
+  if (arg_pos("-epoch-shuffle", argc, argv) > 0) o.epoch_shuffle = true;
+  num("-size", o.layer);
+  for (const char* f : {"-epoch", "-epochs", "-iter", "-iters"})
+    if ((i = arg_pos(f, argc, argv)) > 0 && i + 1 < argc) {
+      o.epochs = atoi(argv[i + 1]);
+      break;
+    }
+  if ((i = arg_pos("-alpha", argc, argv)) > 0 && i + 1 < argc) o.alpha = (float)atof(argv[i + 1]);
+  num("-window", o.window);
+  if ((i = arg_pos("-sample", argc, argv)) > 0 && i + 1 < argc) o.sample = (float)atof(argv[i + 1]);
+  for (const char* f : {"-negative", "-negatives"})
+    if ((i = arg_pos(f, argc, argv)) > 0 && i + 1 < argc) {
+      o.negative = atoi(argv[i + 1]);
+      break;
+    }
+  for (const char* f : {"-thread", "-threads"})
+    if ((i = arg_pos(f, argc, argv)) > 0 && i + 1 < argc) {
+      o.threads = atoi(argv[i + 1]);
+      break;
+    }
+  num("-kernel-batch-size", o.kernel_batch);
+  num("-streams", o.streams);
+  for (const char* f : {"-msl", "-max-sentence-length"})
+    if ((i = arg_pos(f, argc, argv)) > 0 && i + 1 < argc) {
+      o.msl = atoi(argv[i + 1]);
+      break;
+    }
+  str("-precision", o.precision);
+  num("-device-sampling", o.device_sampling);
+  num("-exact-sigmoid", o.exact_sigmoid);
+  num("-device", o.device);
+  if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = strtoull(argv[i + 1], nullptr, 10);
+  if (o.debug > 0) {
+    fprintf(stderr, "** FILES & I/O **\n");
+    if (!o.do_export && !o.output.empty()) fprintf(stderr, "!!\tProcess will train but will not export a model!\n");
+    else if (!o.output.empty()) fprintf(stderr, "Trained model will export to file: %s\n", o.output.c_str());
+    else fprintf(stderr, "!!\tPreprocessing only -- no training.\n");
+    if (!o.read_cache.empty()) fprintf(stderr, "Corpus (serialized cache): %s\n", o.read_cache.c_str());
+    else fprintf(stderr, "Corpus (file): %s\n", o.train.c_str());
+    fprintf(stderr, "\n** TRAINING PARAMETERS **\nlayer size: %d\ntrain for %d epochs\n", o.layer, o.epochs);
+    fprintf(stderr, "initial learning rate: %.5f\nwindow size: %d\nsubsampling rate: %lf\n", o.alpha, o.window,
+            (double)o.sample);
+    fprintf(stderr, "number of negative samples: %d\n", o.negative);
+    fprintf(stderr, "\n** PARALLEL UTILIZATION **\n#CPU threads: %d\nGPU kernel batch size: %d\n", o.threads,
+            o.kernel_batch);
+    fprintf(stderr, "GPU streams per CPU thread: %d\nprecision: %s\ndevice sampling: %d\n\n", o.streams,
+            o.precision.c_str(), o.device_sampling);
+  }
+  return o;
+}
+
+double now_s() {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC_RAW, &t);
+  return t.tv_sec + t.tv_nsec / 1e9;
+}
+
+void die(const char* what, w2v_ctx* ctx, int rc) {
+  fprintf(stderr, "%s failed (%d): %s\n", what, rc, w2v_last_error(ctx));
+  exit(1);
+}
+
+// ------------------------------------------------------------------------------------
+// host batching loop: FULL-W2V-driver.cu:80-354
+// ------------------------------------------------------------------------------------
+struct Shared {
+  const Options* o;
+  const Corpus* corpus;
+  const std::vector<int32_t>* unigram;
+  const std::vector<float>* subs;
+  w2v_ctx* ctx;
+  uint64_t train_words;
+  int64_t n_sent;
+  pthread_mutex_t mu;
+  pthread_barrier_t bar;
+  uint64_t wca = 0;
+  float alpha = 0.025f;
+  double t_start = 0, t_end = 0;
+  Corpus* shuffle_target;
+  std::atomic<int> failed{0};
+};
+
+struct ThreadArg {
+  Shared* sh;
+  int id;
+};
+
+void* train_thread(void* p) {
+  ThreadArg* ta = (ThreadArg*)p;
+  Shared& sh = *ta->sh;
+  const Options& o = *sh.o;
+  const int id = ta->id, K = o.negative, msl = o.msl;
+  {  // one core per thread (driver.cu:97-100), within the allowed set
+    cpu_set_t allowed, set;
+    if (sched_getaffinity(0, sizeof(allowed), &allowed) == 0) {
+      int n = 0, want = id % std::max(1, CPU_COUNT(&allowed));
+      for (int c = 0; c < CPU_SETSIZE; c++)
+        if (CPU_ISSET(c, &allowed) && n++ == want) {
+          CPU_ZERO(&set);
+          CPU_SET(c, &set);
+          sched_setaffinity(0, sizeof(set), &set);
+          break;
+        }
+    }
+  }
+  const int64_t per = sh.n_sent / o.threads, left = sh.n_sent % o.threads;
+  int64_t beg = 0;
+  for (int64_t i = 0; i < id; i++) beg += per + (i < left ? 1 : 0);
+  const int64_t end = beg + per + (id < left ? 1 : 0);
+  const std::vector<int32_t>& stream = sh.corpus->stream;
+  const std::vector<int64_t>& start = sh.corpus->start;
+  const int32_t* uni = sh.unigram->data();
+  const float* subs = sh.subs->empty() ? nullptr : sh.subs->data();
+  uint64_t next_random = (uint64_t)id, word_count = 0, last_word_count = 0;
+  int64_t ss = beg, cur = 0;
+  std::vector<int32_t> btok, bneg;
+  std::vector<int64_t> boff(1, 0);
+  std::vector<float> balpha;
+  int cur_stream = 0;
+  auto submit = [&]() {
+    if (balpha.empty()) return;
+    const int rc = w2v_submit_csr(sh.ctx, id * o.streams + cur_stream, btok.data(), boff.data(),
+                                  (int64_t)balpha.size(), K > 0 ? bneg.data() : nullptr, balpha.data());
+    if (rc) {
+      fprintf(stderr, "w2v_submit_csr failed (%d): %s\n", rc, w2v_last_error(sh.ctx));
+      sh.failed = 1;
+    }
+    btok.clear();
+    bneg.clear();
+    boff.assign(1, 0);
+    balpha.clear();
+    cur_stream = (cur_stream + 1) % o.streams;
+  };
+  auto merge_words = [&]() {
+    const uint64_t diff = word_count - last_word_count;
+    pthread_mutex_lock(&sh.mu);
+    sh.wca += diff;
+    const uint64_t wca = sh.wca;
+    pthread_mutex_unlock(&sh.mu);
+    last_word_count = word_count;
+    return wca;
+  };
+  pthread_barrier_wait(&sh.bar);
+  if (id == 0) sh.t_start = now_s();
+  for (int local_epoch = o.epochs; local_epoch > 0 && !sh.failed;) {
+    if (word_count - last_word_count > 10000) {  // driver.cu:168-190
+      const uint64_t wca = merge_words();
+      if (o.debug > 0 && id == 0) {
+        printf("%cAlpha: %f  Progress: %.2f%%  Words/sec: %.2fk", 13, sh.alpha,
+               wca / (float)((uint64_t)o.epochs * sh.train_words + 1) * 100,
+               (wca / 1000.0) / (now_s() - sh.t_start));
+        fflush(stdout);
+      }
+      float a = o.alpha * (1 - wca / (float)((uint64_t)o.epochs * sh.train_words + 1));
+      if (a < o.alpha * 0.0001f) a = o.alpha * 0.0001f;
+      sh.alpha = a;
+    }
+    // assemble one GPU sentence (driver.cu:193-229)
+    int len = 0;
+    while (len < msl && ss < end) {
+      if (cur >= msl) {
+        ss++;
+        cur = 0;
+        if (ss >= end) break;
+      }
+      const int32_t w = stream[start[ss] + cur];
+      cur++;
+      word_count++;
+      if (w == 0) {
+        ss++;
+        cur = 0;
+        break;
+      }
+      if (o.sample > 0) {
+        next_random = next_random * 25214903917ull + 11;
+        if (subs[w] < (next_random & 0xFFFF) / 65536.f) continue;
+      }
+      btok.push_back(w);
+      for (int k = 0; k < K; k++) {
+        next_random = next_random * 25214903917ull + 11;
+        bneg.push_back(uni[(next_random >> 16) % (uint64_t)kUnigramTableSize]);
+      }
+      len++;
+    }
+    if (ss >= end) {
+      // epoch end (driver.cu:233-277): the partial sentence is dropped, as in the reference
+      btok.resize(boff.back());
+      bneg.resize(boff.back() * K);
+      merge_words();
+      word_count = last_word_count = 0;
+      ss = beg;
+      cur = 0;
+      local_epoch--;
+      if (o.epoch_shuffle) {
+        pthread_barrier_wait(&sh.bar);
+        if (id == 0 && local_epoch > 0) sh.shuffle_target->shuffle();
+        pthread_barrier_wait(&sh.bar);
+      }
+      continue;
+    }
+    boff.push_back((int64_t)btok.size());
+    balpha.push_back(sh.alpha);
+    if ((int)balpha.size() == o.kernel_batch) submit();
+  }
+  submit();
+  pthread_barrier_wait(&sh.bar);
+  for (int s = 0; s < o.streams; s++) w2v_sync(sh.ctx, id * o.streams + s);
+  pthread_barrier_wait(&sh.bar);
+  if (id == 0) sh.t_end = now_s();
+  return nullptr;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Options o = parse(argc, argv);
+  Vocab vocab;
+  vocab.min_count = o.min_count;
+  Corpus corpus;
+  // SetupNetwork (FULL-W2V-driver.cu:356-425)
+  if (!o.train.empty()) {
+    MappedFile f;
+    if (!f.open(o.train)) {
+      printf("ERROR: Training data not found!\n");
+      return 1;
+    }
+    if (o.debug > 0) printf("Train file size = %lluK\n", (unsigned long long)(f.size / 1000));
+    if (!o.read_vocab.empty()) {
+      if (!vocab.read(o.read_vocab)) {
+        printf("Vocabulary file \"%s\" not found!\n", o.read_vocab.c_str());
+        return 1;
+      }
+    } else {
+      vocab.learn(f);
+    }
+    if (o.recount) vocab.recount(f);
+    if (!o.save_vocab.empty()) vocab.save(o.save_vocab);
+    if (!o.read_cache.empty()) {
+      if (!corpus.read_cache(o.read_cache, o.msl, o.cache_eos)) {
+        printf("Corpus cache file \"%s\" not found!\n", o.read_cache.c_str());
+        return 1;
+      }
+    } else {
+      corpus.build(vocab, f, o.msl, o.cache_eos);
+    }
+  } else {
+    if (o.read_vocab.empty()) {
+      printf("ERROR: No vocabulary or training file to generate vocabulary from! Please specify -train or -read-vocab\n");
+      return 1;
+    }
+    if (!vocab.read(o.read_vocab)) {
+      printf("Vocabulary file \"%s\" not found!\n", o.read_vocab.c_str());
+      return 1;
+    }
+    if (!o.save_vocab.empty()) vocab.save(o.save_vocab);
+    if (o.read_cache.empty()) {
+      printf("ERROR: No corpus cache or training file to learn from! Please specify -train or -read-corpus-cache\n");
+      return 1;
+    }
+    if (!corpus.read_cache(o.read_cache, o.msl, o.cache_eos)) {
+      printf("Corpus cache file \"%s\" not found!\n", o.read_cache.c_str());
+      return 1;
+    }
+  }
+  if (!corpus.train_words) corpus.train_words = vocab.train_words;
+  if (!o.save_cache.empty()) corpus.save_cache(o.save_cache, o.msl);
+  if (!o.save_corpus.empty()) corpus.save_text(o.save_corpus, vocab, o.msl);
+  const int64_t n_sent = (int64_t)corpus.start.size();
+  if (o.debug > 1)
+    printf("%lld sentences spread across %d CPU threads == %lld sentences/thread\n", (long long)n_sent,
+           o.threads, (long long)(n_sent / std::max(1, o.threads)));
+  if (o.output.empty()) return 0;
+  if (o.threads < 1 || o.streams < 1 || o.threads * o.streams > 64) {
+    fprintf(stderr, "-threads x -streams must be between 1 and 64\n");
+    return 1;
+  }
+
+  // InitNet (improved_driver_functions.cpp:1-83)
+  const int V = (int)vocab.words.size(), d = o.layer;
+  const bool bf = o.precision == "bf16";
+  if (!bf && o.precision != "fp32") {
+    fprintf(stderr, "-precision must be fp32 or bf16\n");
+    return 1;
+  }
+  w2v_params p{};
+  p.vocab_size = V;
+  p.layer_size = d;
+  p.window = o.window;
+  p.negative = o.negative;
+  p.max_sentence_length = o.msl;
+  p.storage = bf ? W2V_STORAGE_BF16 : W2V_STORAGE_FP32;
+  p.kernel = bf ? W2V_KERNEL_MFMA : W2V_KERNEL_PLAIN;
+  p.sigmoid = o.exact_sigmoid ? W2V_SIGMOID_EXACT : W2V_SIGMOID_TABLE;
+  p.mode = W2V_MODE_HOGWILD;
+  p.device = o.device;
+  p.streams = o.device_sampling ? 1 : o.threads * o.streams;
+  p.seed = o.seed;
+  w2v_ctx* ctx = nullptr;
+  int rc = w2v_create(&p, &ctx);
+  if (rc) {
+    fprintf(stderr, "w2v_create failed (%d): %s\n", rc, w2v_last_error(nullptr));
+    return 1;
+  }
+  if (o.debug > 0) fprintf(stderr, "Initializing network\n");
+  if ((rc = w2v_init_model(ctx))) die("w2v_init_model", ctx, rc);
+  std::vector<uint64_t> cn(V);
+  for (int i = 0; i < V; i++) cn[i] = vocab.words[i].cn;
+  std::vector<int32_t> unigram(kUnigramTableSize);
+  w2v_unigram_table(cn.data(), V, unigram.data(), kUnigramTableSize, 0);
+  std::vector<float> subs;
+  if (o.sample > 0) {
+    subs.resize(V);
+    w2v_subsample_probs(cn.data(), V, o.sample, vocab.train_words, subs.data());
+  }
+  if (o.pre_shuffle) corpus.shuffle();
+  if (o.debug > 0) fprintf(stderr, "Network ready\n");
+
+  uint64_t words = 0;
+  double t0 = 0, t1 = 0;
+  if (o.device_sampling) {
+    // corpus sentences in CSR: tokens of each sentence up to </s> or -msl
+    std::vector<int32_t> tok;
+    std::vector<int64_t> off(1, 0);
+    tok.reserve(corpus.stream.size());
+    for (int64_t s = 0; s < n_sent; s++) {
+      for (int w = 0; w < o.msl; w++) {
+        const int32_t t = corpus.stream[corpus.start[s] + w];
+        if (t == 0) break;
+        tok.push_back(t);
+      }
+      off.push_back((int64_t)tok.size());
+    }
+    if ((rc = w2v_tables_upload(ctx, unigram.data(), kUnigramTableSize, subs.empty() ? nullptr : subs.data())))
+      die("w2v_tables_upload", ctx, rc);
+    if ((rc = w2v_set_schedule(ctx, o.alpha, o.epochs, vocab.train_words))) die("w2v_set_schedule", ctx, rc);
+    if ((rc = w2v_corpus_upload(ctx, tok.data(), (int64_t)tok.size(), off.data(), n_sent)))
+      die("w2v_corpus_upload", ctx, rc);
+    if ((rc = w2v_sync_all(ctx))) die("w2v_sync_all", ctx, rc);
+    t0 = now_s();
+    const int64_t chunk_tokens = 100000000;
+    for (int e = 0; e < o.epochs; e++) {
+      for (int64_t sb = 0; sb < n_sent;) {
+        int64_t se = sb + 1;
+        while (se < n_sent && off[se + 1] - off[sb] <= chunk_tokens) se++;
+        if ((rc = w2v_train_device(ctx, 0, e, sb, se))) die("w2v_train_device", ctx, rc);
+        sb = se;
+      }
+      if (o.debug > 0) {
+        if ((rc = w2v_sync_all(ctx))) die("w2v_sync_all", ctx, rc);
+        w2v_stats st;
+        w2v_get_stats(ctx, &st);
+        printf("%cEpoch %d/%d  Words/sec: %.2fk", 13, e + 1, o.epochs, st.words_read / 1000.0 / (now_s() - t0));
+        fflush(stdout);
+      }
+      if (o.epoch_shuffle && e + 1 < o.epochs && o.debug > 0)
+        fprintf(stderr, "\n-epoch-shuffle is not applied on the device-sampling path\n");
+    }
+    if ((rc = w2v_sync_all(ctx))) die("w2v_sync_all", ctx, rc);
+    t1 = now_s();
+    w2v_stats st;
+    w2v_get_stats(ctx, &st);
+    words = st.words_read;
+  } else {
+    Shared sh;
+    sh.o = &o;
+    sh.corpus = &corpus;
+    sh.unigram = &unigram;
+    sh.subs = &subs;
+    sh.ctx = ctx;
+    sh.train_words = vocab.train_words;
+    sh.n_sent = n_sent;
+    sh.alpha = o.alpha;
+    sh.shuffle_target = &corpus;
+    pthread_mutex_init(&sh.mu, nullptr);
+    pthread_barrier_init(&sh.bar, nullptr, o.threads);
+    std::vector<pthread_t> th(o.threads);
+    std::vector<ThreadArg> args(o.threads);
+    for (int i = 0; i < o.threads; i++) {
+      args[i] = {&sh, i};
+      pthread_create(&th[i], nullptr, train_thread, &args[i]);
+    }
+    for (int i = 0; i < o.threads; i++) pthread_join(th[i], nullptr);
+    if (sh.failed) return 1;
+    words = sh.wca;
+    t0 = sh.t_start;
+    t1 = sh.t_end;
+  }
+  // stdout contract of the reference (FULL-W2V-driver.cu:450-451)
+  printf("\nOverall Words Per Second: %.4f\n", words / (t1 - t0));
+  printf("Overall Words: %llu\nOverall time: %f\n", (unsigned long long)words, t1 - t0);
+  if (o.do_export) {
+    std::vector<float> syn0((size_t)V * d);
+    if ((rc = w2v_model_download(ctx, syn0.data(), nullptr))) die("w2v_model_download", ctx, rc);
+    if (o.debug > 0) fprintf(stderr, "Saving model to: %s\n", o.output.c_str());
+    if (!save_vectors(o.output, vocab, syn0.data(), d, o.binary != 0)) {
+      fprintf(stderr, "cannot write %s\n", o.output.c_str());
+      return 1;
+    }
+  }
+  w2v_destroy(ctx);
+  return 0;
+}

@@ -1,0 +1,302 @@
+// Vocabulary, corpus stream and file formats of the host pipeline. Each routine
+// restates the reference routine named in text.h; deviations are noted inline.
+#include "text.h"
+
+#include <fcntl.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+
+namespace w2vhost {
+
+bool MappedFile::open(const std::string& path) {
+  fd = ::open(path.c_str(), O_RDONLY);
+  if (fd < 0) return false;
+  struct stat st;
+  if (fstat(fd, &st) < 0) return false;
+  size = (uint64_t)st.st_size;
+  if (size > 0) {
+    void* p = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (p == MAP_FAILED) return false;
+    data = (const char*)p;
+  }
+  return true;
+}
+
+MappedFile::~MappedFile() {
+  if (data) munmap((void*)data, size);
+  if (fd >= 0) ::close(fd);
+}
+
+void read_word(char* word, const char* buf, uint64_t size, uint64_t* cur) {
+  int len = 0;
+  char ch = 0;
+  uint64_t pos = *cur;
+  while (pos < size) {
+    ch = buf[pos++];
+    if (ch == 13) continue;
+    if (ch == '\n') {
+      if (len > 0) {
+        pos--;  // leave the newline for the next call, which returns </s>
+        break;
+      }
+      strcpy(word, "</s>");
+      *cur = pos;
+      return;
+    }
+    if (ch == ' ' || ch == '\t') {
+      if (len > 0) break;
+      continue;
+    }
+    word[len++] = ch;
+    if (len >= kMaxString - 1) break;
+  }
+  word[len] = 0;
+  if (pos == size || ch == ' ' || ch == '\t' || ch == '\n') {
+    *cur = pos;
+    return;
+  }
+  // truncated word: skip to the next separator
+  while (pos < size && ch != ' ' && ch != '\t' && ch != '\n') ch = buf[pos++];
+  *cur = pos - 1;
+}
+
+Vocab::Vocab() : hash(kVocabHashSize, -1) {}
+
+uint32_t Vocab::word_hash(const char* w) {
+  uint32_t h = 0;
+  for (const unsigned char* p = (const unsigned char*)w; *p; p++) h = h * 257u + (uint32_t)(signed char)*p;
+  return h % (uint32_t)kVocabHashSize;
+}
+
+int Vocab::search(const char* w) const {
+  uint32_t h = word_hash(w);
+  for (;;) {
+    const int32_t i = hash[h];
+    if (i == -1) return -1;
+    if (words[i].word == w) return i;
+    h = (h + 1) % (uint32_t)kVocabHashSize;
+  }
+}
+
+int Vocab::add(const char* w) {
+  Entry e;
+  e.word.assign(w, strnlen(w, kMaxString - 1));
+  words.push_back(e);
+  uint32_t h = word_hash(w);
+  while (hash[h] != -1) h = (h + 1) % (uint32_t)kVocabHashSize;
+  hash[h] = (int32_t)words.size() - 1;
+  return (int)words.size() - 1;
+}
+
+void Vocab::rehash() {
+  std::fill(hash.begin(), hash.end(), -1);
+  for (size_t i = 0; i < words.size(); i++) {
+    uint32_t h = word_hash(words[i].word.c_str());
+    while (hash[h] != -1) h = (h + 1) % (uint32_t)kVocabHashSize;
+    hash[h] = (int32_t)i;
+  }
+}
+
+void Vocab::sort_and_prune() {
+  // </s> stays at index 0; the rest by count, descending. The reference uses qsort; a
+  // stable sort keeps equal counts in first-seen order (what glibc's merge sort does).
+  if (words.size() > 1)
+    std::stable_sort(words.begin() + 1, words.end(),
+                     [](const Entry& a, const Entry& b) { return a.cn > b.cn; });
+  size_t keep = words.size();
+  for (size_t i = 1; i < words.size(); i++)
+    if (words[i].cn < (uint64_t)min_count) {
+      keep = i;  // sorted, so every later word is below min_count as well
+      break;
+    }
+  words.resize(keep);
+  train_words = 0;
+  for (auto& e : words) train_words += e.cn;
+  rehash();
+}
+
+void Vocab::reduce() {
+  size_t n = 0;
+  for (size_t i = 0; i < words.size(); i++)
+    if (words[i].cn > min_reduce) words[n++] = words[i];
+  words.resize(n);
+  rehash();
+  min_reduce++;
+}
+
+void Vocab::learn(const MappedFile& f) {
+  char w[kMaxString];
+  train_words = 0;
+  words.clear();
+  std::fill(hash.begin(), hash.end(), -1);
+  add("</s>");
+  for (uint64_t pos = 0; pos < f.size;) {
+    read_word(w, f.data, f.size, &pos);
+    if (pos == f.size) break;  // the word that reaches end of file is not counted
+    train_words++;
+    const int i = search(w);
+    if (i == -1) words[add(w)].cn = 1;
+    else words[i].cn++;
+    if (words.size() > kVocabHashSize * 0.7) reduce();
+  }
+  sort_and_prune();
+  train_words++;
+}
+
+bool Vocab::read(const std::string& path) {
+  MappedFile f;
+  if (!f.open(path)) return false;
+  char w[kMaxString];
+  words.clear();
+  std::fill(hash.begin(), hash.end(), -1);
+  for (uint64_t pos = 0; pos < f.size; pos++) {
+    read_word(w, f.data, f.size, &pos);
+    const int i = add(w);
+    read_word(w, f.data, f.size, &pos);
+    words[i].cn = (uint64_t)atoi(w);
+  }
+  sort_and_prune();
+  return true;
+}
+
+void Vocab::recount(const MappedFile& f) {
+  char w[kMaxString];
+  train_words = 0;
+  for (auto& e : words) e.cn = 0;
+  for (uint64_t pos = 0; pos < f.size;) {
+    read_word(w, f.data, f.size, &pos);
+    const int i = search(w);
+    if (i == -1) continue;
+    words[i].cn++;
+    train_words++;
+  }
+  sort_and_prune();
+}
+
+bool Vocab::save(const std::string& path) const {
+  FILE* fo = fopen(path.c_str(), "wb");
+  if (!fo) return false;
+  for (auto& e : words) fprintf(fo, "%s %d\n", e.word.c_str(), (int)e.cn);
+  fclose(fo);
+  return true;
+}
+
+void Corpus::build(const Vocab& v, const MappedFile& f, int msl, bool cache_eos) {
+  char w[kMaxString];
+  stream.clear();
+  start.clear();
+  uint64_t len = 0;
+  for (uint64_t pos = 0; pos < f.size;) {
+    int idx = -1;
+    while (pos < f.size && idx == -1) {
+      read_word(w, f.data, f.size, &pos);
+      idx = v.search(w);
+    }
+    // An unknown word at end of file yields -1 in the reference (ReadWordIndex), which
+    // would then be used as a row index; it is dropped here.
+    if (idx < 0) continue;
+    if (cache_eos || idx != 0) {
+      if (len == 0) start.push_back((int64_t)stream.size());
+      stream.push_back(idx);
+      len++;
+      if (idx == 0 || len >= (uint64_t)msl) len = 0;
+    }
+  }
+  stream.push_back(0);  // the stream always ends with </s>
+  train_words = v.train_words;
+}
+
+bool Corpus::read_cache(const std::string& path, int msl, bool cache_eos) {
+  MappedFile f;
+  if (!f.open(path)) return false;
+  char w[kMaxString];
+  uint64_t pos = 0;
+  read_word(w, f.data, f.size, &pos);
+  const uint64_t nsent = strtoull(w, nullptr, 10);
+  read_word(w, f.data, f.size, &pos);
+  train_words = strtoull(w, nullptr, 10);
+  pos++;
+  stream.clear();
+  start.clear();
+  stream.reserve(train_words + nsent + 2);
+  start.push_back(0);
+  for (uint64_t trig = 0; pos < f.size;) {
+    read_word(w, f.data, f.size, &pos);
+    if (!strcmp(w, "</s>")) {
+      if (cache_eos) {
+        stream.push_back(0);
+        start.push_back((int64_t)stream.size());
+        trig = 0;
+      }
+    } else {
+      if (trig == (uint64_t)msl) {  // split rule of improved_driver_functions.cpp:378-389
+        stream.push_back(0);
+        start.push_back((int64_t)stream.size());
+        trig = 0;
+      } else {
+        trig++;
+      }
+      stream.push_back(atoi(w));
+    }
+  }
+  // the last sentence pointer is not counted (improved_driver_functions.cpp:399)
+  start.pop_back();
+  stream.push_back(0);
+  return true;
+}
+
+bool Corpus::save_cache(const std::string& path, int msl) const {
+  FILE* fo = fopen(path.c_str(), "w");
+  if (!fo) return false;
+  fprintf(fo, "%llu %llu\n", (unsigned long long)start.size(), (unsigned long long)train_words);
+  for (size_t s = 0; s < start.size(); s++) {
+    int w = 0;
+    for (; w < msl && stream[start[s] + w] != 0; w++) fprintf(fo, "%u ", (unsigned)stream[start[s] + w]);
+    if (w > 0) fprintf(fo, "\n");
+  }
+  fclose(fo);
+  return true;
+}
+
+bool Corpus::save_text(const std::string& path, const Vocab& v, int msl) const {
+  FILE* fo = fopen(path.c_str(), "w");
+  if (!fo) return false;
+  for (size_t s = 0; s < start.size(); s++) {
+    int w = 0;
+    for (; w < msl && stream[start[s] + w] != 0; w++) fprintf(fo, "%s ", v.words[stream[start[s] + w]].word.c_str());
+    if (w > 0) fprintf(fo, "\n");
+  }
+  fclose(fo);
+  return true;
+}
+
+void Corpus::shuffle() {
+  if (start.empty()) return;
+  for (uint64_t i = start.size() - 1; i > 0; i--) {
+    const uint64_t j = (uint64_t)((rand() / (double)RAND_MAX) * (i + 1));
+    std::swap(start[j < start.size() ? j : start.size() - 1], start[i]);
+  }
+}
+
+bool save_vectors(const std::string& path, const Vocab& v, const float* syn0, int d, bool binary) {
+  FILE* fo = fopen(path.c_str(), "wb");
+  if (!fo) return false;
+  fprintf(fo, "%d %d\n", (int)v.words.size(), d);
+  for (size_t a = 0; a < v.words.size(); a++) {
+    fprintf(fo, "%s ", v.words[a].word.c_str());
+    if (binary) fwrite(syn0 + a * d, sizeof(float), d, fo);
+    else
+      for (int b = 0; b < d; b++) fprintf(fo, "%f ", syn0[a * d + b]);
+    fprintf(fo, "\n");
+  }
+  fclose(fo);
+  return true;
+}
+
+}  // namespace w2vhost

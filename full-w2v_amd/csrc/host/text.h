@@ -1,0 +1,77 @@
+// Host data pipeline of the trainer: vocabulary, corpus stream and the reference's
+// file formats (vocab file, corpus cache, exported vectors).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace w2vhost {
+
+constexpr int kMaxString = 800;                // MAX_STRING (FULL-W2V-driver.cu:16)
+constexpr int kVocabHashSize = 30000000;       // vocab_hash_size (FULL-W2V-driver.cu:60)
+
+// A read-only memory-mapped file.
+struct MappedFile {
+  const char* data = nullptr;
+  uint64_t size = 0;
+  int fd = -1;
+  bool open(const std::string& path);
+  ~MappedFile();
+};
+
+// Word reader over a mapped buffer (improved_driver_functions.cpp:86-124): spaces and
+// tabs separate words, '\r' is skipped, a newline yields "</s>", words longer than
+// kMaxString-1 bytes are truncated.
+void read_word(char* word, const char* buf, uint64_t size, uint64_t* pos);
+
+struct Vocab {
+  struct Entry {
+    std::string word;
+    uint64_t cn = 0;
+  };
+  std::vector<Entry> words;
+  std::vector<int32_t> hash;  // open addressing, kVocabHashSize slots
+  int min_count = 5;
+  uint64_t min_reduce = 1;
+  uint64_t train_words = 0;
+
+  Vocab();
+  static uint32_t word_hash(const char* w);  // h = h*257 + c (legacy_driver_functions.cpp:26-31)
+  int search(const char* w) const;
+  int add(const char* w);
+  void sort_and_prune();  // SortVocab (legacy:74-95): </s> stays first, cn < min_count dropped
+  void reduce();          // ReduceVocab (legacy:98-118)
+  void rehash();
+  // LearnVocabFromTrainFile (improved:136-164)
+  void learn(const MappedFile& f);
+  // ReadVocab (improved:167-196); returns false if the file cannot be opened
+  bool read(const std::string& path);
+  // recountVocab (improved:199-223)
+  void recount(const MappedFile& f);
+  // SaveVocab (legacy:120-124): "word count\n"
+  bool save(const std::string& path) const;
+};
+
+// Corpus as one token stream plus sentence start offsets (global_stream /
+// global_sentences of the reference).
+struct Corpus {
+  std::vector<int32_t> stream;
+  std::vector<int64_t> start;
+  uint64_t train_words = 0;
+  // createGlobalStream (improved:244-281)
+  void build(const Vocab& v, const MappedFile& f, int msl, bool cache_eos);
+  // readCorpusCache (improved:320-401); false if the file cannot be opened
+  bool read_cache(const std::string& path, int msl, bool cache_eos);
+  // saveCorpusCache (improved:306-318) and saveCorpus (improved:293-304)
+  bool save_cache(const std::string& path, int msl) const;
+  bool save_text(const std::string& path, const Vocab& v, int msl) const;
+  // shuffleSentences (improved:283-291)
+  void shuffle();
+};
+
+// saveModel (improved:225-242): "V d\n" then "word " + d values (text "%f " or raw
+// float32) + "\n" per word.
+bool save_vectors(const std::string& path, const Vocab& v, const float* syn0, int d, bool binary);
+
+}  // namespace w2vhost

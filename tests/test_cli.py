@@ -1,0 +1,135 @@
+"""The full-w2v command line: reference flags, file formats, exit codes, stdout contract.
+
+Expected vocabulary and corpus-cache contents are hand-traced from the reference's
+routines (LearnVocabFromTrainFile, SortVocab, createGlobalStream, saveCorpusCache:
+improved_driver_functions.cpp:136-164,244-318, legacy_driver_functions.cpp:74-95).
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "full-w2v_amd", "full-w2v")
+TEXT = "the cat sat on the mat\nthe dog sat\r\non the log the end\n"
+
+
+def run(args, cwd, check=None):
+    r = subprocess.run([CLI] + args + ["-debug", "0"], cwd=cwd, capture_output=True, text=True, timeout=600)
+    if check is not None:
+        assert r.returncode == check, r.stdout + r.stderr
+    return r
+
+
+@pytest.fixture
+def corpus_dir(tmp_path):
+    (tmp_path / "t.txt").write_text(TEXT)
+    return tmp_path
+
+
+def test_vocab_and_cache_formats(corpus_dir):
+    run(["-train", "t.txt", "-min-count", "2", "-save-vocab", "v.txt", "-save-corpus-cache", "c.txt",
+         "-save-corpus", "s.txt", "-cache-eos"], corpus_dir, check=0)
+    # the final </s> reaches end of file and is not counted; ties keep first-seen order
+    assert (corpus_dir / "v.txt").read_text() == "</s> 2\nthe 5\nsat 2\non 2\n"
+    # header: sentences, train_words (= 11 counted + 1)
+    assert (corpus_dir / "c.txt").read_text() == "3 12\n1 2 3 1 \n1 2 \n3 1 1 \n"
+    assert (corpus_dir / "s.txt").read_text() == "the sat on the \nthe sat \non the the \n"
+
+
+def test_msl_split_without_eos(corpus_dir):
+    run(["-train", "t.txt", "-min-count", "1", "-save-corpus-cache", "c.txt", "-msl", "4"], corpus_dir, check=0)
+    lines = (corpus_dir / "c.txt").read_text().splitlines()
+    # 14 known non-</s> tokens in chunks of 4
+    assert lines[0].split()[0] == "4"
+    assert [len(x.split()) for x in lines[1:]] == [4, 4, 4, 2]
+
+
+def test_read_vocab_and_cache_round_trip(corpus_dir):
+    run(["-train", "t.txt", "-min-count", "1", "-save-vocab", "v.txt", "-save-corpus-cache", "c.txt",
+         "-cache-eos"], corpus_dir, check=0)
+    run(["-read-vocab", "v.txt", "-read-corpus-cache", "c.txt", "-cache-eos", "-min-count", "1", "-save-vocab", "v2.txt",
+         "-save-corpus-cache", "c2.txt"], corpus_dir, check=0)
+    assert (corpus_dir / "v.txt").read_text() == (corpus_dir / "v2.txt").read_text()
+    a = (corpus_dir / "c.txt").read_text().splitlines()
+    b = (corpus_dir / "c2.txt").read_text().splitlines()
+    assert a[1:] == b[1:]
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-train", "missing.txt"], "Training data not found"),
+    (["-read-vocab", "missing.txt"], "not found"),
+    (["-read-vocab", "v.txt"], "No corpus cache or training file"),
+    (["-output", "o.txt"], "No vocabulary or training file"),
+    (["-train", "t.txt", "-read-corpus-cache", "missing.txt"], "not found"),
+])
+def test_error_exits(corpus_dir, args, msg):
+    (corpus_dir / "v.txt").write_text("</s> 1\nthe 5\n")
+    r = run(args, corpus_dir)
+    assert r.returncode == 1 and msg in r.stdout + r.stderr
+
+
+def test_help_exits_zero(corpus_dir):
+    r = subprocess.run([CLI, "-help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "-kernel-batch-size" in r.stdout and "-precision" in r.stdout
+
+
+def words_text(tmp_path, n_lines=3000, seed=0):
+    rng = np.random.default_rng(seed)
+    vocab = [f"w{i}" for i in range(300)]
+    p = 1.0 / np.arange(1, 301)
+    p /= p.sum()
+    lines = []
+    for _ in range(n_lines):
+        L = int(rng.integers(3, 30))
+        lines.append(" ".join(vocab[i] for i in rng.choice(300, L, p=p)))
+    (tmp_path / "big.txt").write_text("\n".join(lines) + "\n")
+
+
+def parse_contract(out):
+    wps = float(re.search(r"Overall Words Per Second: ([0-9.]+)", out).group(1))
+    words = int(re.search(r"Overall Words: ([0-9]+)", out).group(1))
+    secs = float(re.search(r"Overall time: ([0-9.]+)", out).group(1))
+    return wps, words, secs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [
+    ["-precision", "fp32"],
+    ["-precision", "bf16"],
+    ["-precision", "fp32", "-device-sampling", "1"],
+    ["-precision", "bf16", "-device-sampling", "1"],
+])
+def test_cli_trains_and_exports(tmp_path, extra):
+    words_text(tmp_path)
+    r = run(["-train", "big.txt", "-output", "vec.txt", "-size", "64", "-window", "3", "-negative", "4",
+             "-epochs", "2", "-threads", "4", "-streams", "2", "-kernel-batch-size", "100",
+             "-cache-eos", "-min-count", "1", "-sample", "1e-3"] + extra, tmp_path, check=0)
+    wps, words, secs = parse_contract(r.stdout)
+    assert words > 0 and wps > 0 and secs > 0
+    txt = (tmp_path / "vec.txt").read_text().splitlines()
+    V, d = map(int, txt[0].split())
+    assert d == 64 and len(txt) == V + 1
+    first = txt[1].split()
+    assert first[0] == "</s>" and len(first) == 65
+    vals = np.array([[float(x) for x in line.split()[1:]] for line in txt[1:]])
+    assert np.isfinite(vals).all() and np.abs(vals).max() > 0
+
+
+@pytest.mark.gpu
+def test_cli_binary_export(tmp_path):
+    words_text(tmp_path, 500)
+    run(["-train", "big.txt", "-output", "vec.bin", "-binary", "1", "-size", "32", "-epochs", "1",
+         "-threads", "2", "-min-count", "1"], tmp_path, check=0)
+    raw = (tmp_path / "vec.bin").read_bytes()
+    head, rest = raw.split(b"\n", 1)
+    V, d = map(int, head.split())
+    pos = 0
+    for _ in range(V):
+        sp = rest.index(b" ", pos)
+        vec = np.frombuffer(rest[sp + 1:sp + 1 + 4 * d], np.float32)
+        assert np.isfinite(vec).all()
+        pos = sp + 1 + 4 * d + 1
+    assert pos == len(rest)
