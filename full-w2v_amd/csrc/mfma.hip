@@ -86,10 +86,22 @@ __device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, flo
   __builtin_amdgcn_global_atomic_fadd_v2bf16((bf16x2_t*)p, v);
 }
 
+// Table sigmoid with the reference's buckets (index rule of FULL-W2V-kernels.cu:92-99),
+// the bucket value evaluated in registers with the generator's formula
+// (generate_expTable_h.cpp:48-51: sigma((2i/1000-1)*6), ends 0 and 1) instead of a
+// per-lane table gather; nexpTable[j] equals expTable[999-j], so one formula serves both.
+// Values agree with the stored table to a few ulp (device exp).
+__device__ __forceinline__ float table_value(int i) {
+  if (i <= 0) return 0.0f;
+  if (i >= 999) return 1.0f;
+  const float v = __expf((i / 1000.0f * 2 - 1) * 6);
+  return v / (v + 1);
+}
+
 __device__ __forceinline__ float tile_grad(float f, bool positive, float alpha, const UpdateArgs& a) {
   if (a.sig_exact) return positive ? alpha / (1.0f + expf(f)) : -alpha / (1.0f + expf(-f));
   const int idx = sigmoid_index(f);
-  return positive ? alpha * a.nexp_t[idx] : -alpha * a.exp_t[idx];
+  return positive ? alpha * table_value(999 - idx) : -alpha * table_value(idx);
 }
 
 template <int NP, int NTB>
@@ -209,22 +221,78 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
       ring_tok = (lane == p) ? tok : ring_tok;
       load_row(p, tok);
     }
+    // Hogwild mode loads the next centre's target rows and the next entering context
+    // row one centre ahead (packed bf16 pairs); deterministic mode loads in order.
+    const bool pipe = !a.fence && !delta;
+    auto entering_token = [&](int pos) -> int {
+      if (pos - chunk0 >= 64) {
+        chunk0 += 64;
+        tokbuf = (chunk0 + lane < L) ? a.tokens[base + chunk0 + lane] : 0;
+      }
+      return readlane_i(tokbuf, pos - chunk0);
+    };
+    int ntk[NTB];
+    uint32_t nT[NTB][NP], nE[NP];
+    auto targets_of = [&](int w, int slot_of_w, int* ids) {
+#pragma unroll
+      for (int k = 0; k < NTB; k++) {
+        ids[k] = 0;
+        if (k < NT) ids[k] = (k == 0) ? readlane_i(ring_tok, slot_of_w) : negative(w, k - 1);
+      }
+    };
+    auto fetch_pairs = [&](const uint16_t* m, int row, uint32_t* dst) {
+#pragma unroll
+      for (int e = 0; e < NP; e++) {
+        const int j = 2 * lane + 128 * e;
+        dst[e] = (j < P) ? ((const uint32_t*)(m + (size_t)row * P))[j >> 1] : 0u;
+      }
+    };
+    if (pipe) {
+      targets_of(0, 0, ntk);
+#pragma unroll
+      for (int k = 0; k < NTB; k++)
+        if (k < NT) fetch_pairs(a.bsyn1, ntk[k], nT[k]);
+    }
 
     int cs = 0;
     for (int w = 0; w < L; w++) {
+      const int enter_pos = w + W + 1;
+      int enter_tok = 0;
+      if (enter_pos < L) {
+        enter_tok = entering_token(enter_pos);
+        if (pipe) fetch_pairs(a.bsyn0, enter_tok, nE);
+      }
       // ---- targets: fresh rows, FP32 in registers and bf16 in the MFMA stage
       int tk[NTB];
       float Tf[NTB][NP][2];
+      uint32_t cT[NTB][NP];
+      if (pipe) {
+#pragma unroll
+        for (int k = 0; k < NTB; k++) {
+          tk[k] = ntk[k];
+#pragma unroll
+          for (int e = 0; e < NP; e++) cT[k][e] = nT[k][e];
+        }
+        if (w + 1 < L) {  // next centre's targets
+          targets_of(w + 1, cs + 1 == R ? 0 : cs + 1, ntk);
+#pragma unroll
+          for (int k = 0; k < NTB; k++)
+            if (k < NT) fetch_pairs(a.bsyn1, ntk[k], nT[k]);
+        }
+      } else {
+        targets_of(w, cs, tk);
+#pragma unroll
+        for (int k = 0; k < NTB; k++)
+          if (k < NT) fetch_pairs(a.bsyn1, tk[k], cT[k]);
+      }
 #pragma unroll
       for (int k = 0; k < NTB; k++) {
-        tk[k] = 0;
-        if (k < NT) tk[k] = (k == 0) ? readlane_i(ring_tok, cs) : negative(w, k - 1);
 #pragma unroll
         for (int e = 0; e < NP; e++) {
           const int j = 2 * lane + 128 * e;
           float x0 = 0.0f, x1 = 0.0f;
           if (k < NT && j < P) {
-            const uint32_t pr = ((const uint32_t*)(a.bsyn1 + (size_t)tk[k] * P))[j >> 1];
+            const uint32_t pr = cT[k][e];
             x0 = bf_lo(pr);
             x1 = bf_hi(pr);
             if (delta) {
@@ -342,13 +410,20 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
         if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
       }
       if (enter < L) {
-        if (enter - chunk0 >= 64) {
-          chunk0 += 64;
-          tokbuf = (chunk0 + lane < L) ? a.tokens[base + chunk0 + lane] : 0;
+        ring_tok = (lane == slot) ? enter_tok : ring_tok;
+        if (pipe) {
+          float* uf = Uf + slot * ly.PU;
+#pragma unroll
+          for (int e = 0; e < NP; e++) {
+            const int j = 2 * lane + 128 * e;
+            if (j < P) {
+              ((uint32_t*)U0)[(slot * P + j) >> 1] = nE[e];
+              *(float2*)(uf + j) = make_float2(bf_lo(nE[e]), bf_hi(nE[e]));
+            }
+          }
+        } else {
+          load_row(slot, enter_tok);
         }
-        const int tok = readlane_i(tokbuf, enter - chunk0);
-        ring_tok = (lane == slot) ? tok : ring_tok;
-        load_row(slot, tok);
       }
       __builtin_amdgcn_wave_barrier();
       cs = (cs + 1 == R) ? 0 : cs + 1;

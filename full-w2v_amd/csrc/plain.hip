@@ -120,6 +120,25 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       return readlane_i(negbuf, (w - nw0) * K + k);
     };
 
+    // Hogwild mode loads the next target row and the next entering context row one
+    // step ahead; deterministic mode loads in order after its fences.
+    const bool pipe = !a.fence;
+    auto fetch = [&](const float* m, const float* dm, int row, float* dst) {
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        float v = m[(size_t)row * P + e * 64 + lane];
+        if (dm) v += dm[(size_t)row * P + e * 64 + lane];
+        dst[e] = v;
+      }
+    };
+    auto entering_token = [&](int pos) -> int {
+      if (pos - chunk0 >= 64) {
+        chunk0 += 64;
+        tokbuf = (chunk0 + lane < L) ? a.tokens[base + chunk0 + lane] : 0;
+      }
+      return readlane_i(tokbuf, pos - chunk0);
+    };
+
     // preload positions 0..W (kernels.cu:139-145)
     const int npre = L < W + 1 ? L : W + 1;
     for (int p = 0; p < npre; p++) {
@@ -127,23 +146,36 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       ring_tok = (lane == p) ? tok : ring_tok;
       load_row(p, tok);
     }
+    float nT[E], nE[E];  // prefetched next target row / next entering row
+    int t_next = readlane_i(ring_tok, 0);
+    if (pipe) fetch(a.syn1, a.rd1, t_next, nT);
 
     int cs = 0;  // ring slot of the centre
     for (int w = 0; w < L; w++) {
       const int lo = w - W < 0 ? 0 : w - W;
       const int hi = w + W > L - 1 ? L - 1 : w + W;
       const int lo_slot0 = cs - (w - lo) < 0 ? cs - (w - lo) + R : cs - (w - lo);
+      const int enter_pos = w + W + 1;
+      int enter_tok = 0;
+      if (enter_pos < L) {
+        enter_tok = entering_token(enter_pos);
+        if (pipe) fetch(a.syn0, a.rd0, enter_tok, nE);
+      }
       for (int k = 0; k <= K; k++) {
-        const int t = (k == 0) ? readlane_i(ring_tok, cs) : negative(w, k - 1);
-        const float* trow = a.syn1 + (size_t)t * P;
+        const int t = t_next;
         float T0[E], DT[E];
+        if (pipe) {
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-          float v = trow[e * 64 + lane];
-          if (a.rd1) v += a.rd1[(size_t)t * P + e * 64 + lane];
-          T0[e] = v;
-          DT[e] = 0.0f;
+          for (int e = 0; e < E; e++) T0[e] = nT[e];
+        } else {
+          fetch(a.syn1, a.rd1, t, T0);
         }
+#pragma unroll
+        for (int e = 0; e < E; e++) DT[e] = 0.0f;
+        // next target: the next negative of this centre, or the next centre's word
+        if (k < K) t_next = negative(w, k);
+        else if (w + 1 < L) t_next = readlane_i(ring_tok, cs + 1 == R ? 0 : cs + 1);
+        if (pipe && (k < K || w + 1 < L)) fetch(a.syn1, a.rd1, t_next, nT);
         int slot = lo_slot0;
         for (int c = lo; c <= hi; c++, slot = (slot + 1 == R) ? 0 : slot + 1) {
           if (c == w) continue;
@@ -184,13 +216,18 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
         det_fence(a);
       }
       if (enter < L) {
-        if (enter - chunk0 >= 64) {
-          chunk0 += 64;
-          tokbuf = (chunk0 + lane < L) ? a.tokens[base + chunk0 + lane] : 0;
+        ring_tok = (lane == slot) ? enter_tok : ring_tok;
+        if (pipe) {
+          float* u0 = U0s + (size_t)slot * P;
+          float* dd = Ds + (size_t)slot * P;
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            u0[e * 64 + lane] = nE[e];
+            dd[e * 64 + lane] = 0.0f;
+          }
+        } else {
+          load_row(slot, enter_tok);
         }
-        const int tok = readlane_i(tokbuf, enter - chunk0);
-        ring_tok = (lane == slot) ? tok : ring_tok;
-        load_row(slot, tok);
       }
       cs = (cs + 1 == R) ? 0 : cs + 1;
     }
