@@ -27,7 +27,7 @@ hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t
                           const float* subs_p, uint64_t mixed_seed, int32_t epoch,
                           int32_t* flags, int32_t* pos, int32_t* kept, int64_t* kept_off,
                           void* temp, size_t temp_bytes, int64_t tok_begin, int64_t n,
-                          hipStream_t st);
+                          int64_t key_base, hipStream_t st);
 hipError_t launch_heldout_loss(const float* syn0, const float* syn1, const uint16_t* bsyn0,
                                const uint16_t* bsyn1, int P, int W, int K,
                                const int32_t* tokens, const int64_t* off, int64_t n_sent,
@@ -98,7 +98,7 @@ struct w2v_ctx {
   float alpha0 = 0.025f;
   int32_t epochs = 1;
   uint64_t train_words = 0;
-  uint64_t words_read = 0;
+  std::atomic<uint64_t> words_read{0};
   std::atomic<uint64_t> launches{0};
   std::string err;
 };
@@ -658,7 +658,7 @@ int w2v_train_device(w2v_ctx* c, int32_t stream, int32_t epoch, int64_t sb, int6
   UpdateArgs a = base_args(c);
   HIP_TRY(c, launch_compact(c->d_corpus, c->d_off, sb, se, c->d_subs, a.mixed_seed, epoch,
                             (int32_t*)s.flags.p, (int32_t*)s.pos.p, (int32_t*)s.kept.p,
-                            (int64_t*)s.kept_off.p, s.temp.p, s.temp.cap, tb, n, s.st));
+                            (int64_t*)s.kept_off.p, s.temp.p, s.temp.cap, tb, n, 0, s.st));
   a.tokens = (const int32_t*)s.kept.p;
   a.offsets = (const int64_t*)s.kept_off.p;
   a.neg = nullptr;
@@ -671,6 +671,57 @@ int w2v_train_device(w2v_ctx* c, int32_t stream, int32_t epoch, int64_t sb, int6
   a.total_words = (uint64_t)c->epochs * c->train_words + 1;
   a.epoch_words = (uint64_t)epoch * c->train_words;
   a.n_sent = se - sb;
+  c->words_read += (uint64_t)n;
+  return launch_update(c, s, a);
+}
+
+int w2v_train_chunk(w2v_ctx* c, int32_t stream, int32_t epoch, const int32_t* tokens,
+                    const int64_t* offsets, int64_t ns, int64_t first_token) {
+  int rc = check_stream(c, stream);
+  if (rc) return rc;
+  if (ns < 0 || !offsets || first_token < 0) return fail(c, W2V_ERR_ARG, "invalid chunk");
+  if (ns == 0) return W2V_OK;
+  if (c->p.negative > 0 && !c->d_unigram) return fail(c, W2V_ERR_STATE, "no unigram table uploaded");
+  if (epoch < 0 || epoch >= c->epochs) return fail(c, W2V_ERR_ARG, "epoch outside the schedule");
+  if (offsets[0] != 0) return fail(c, W2V_ERR_ARG, "offsets[0] must be 0");
+  for (int64_t s = 0; s < ns; s++)
+    if (offsets[s + 1] < offsets[s]) return fail(c, W2V_ERR_ARG, "offsets must be non-decreasing");
+  const int64_t n = offsets[ns];
+  if (n > 0 && !tokens) return fail(c, W2V_ERR_ARG, "tokens required");
+  if (n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "at most 2^31-2 tokens per chunk");
+  if ((rc = validate_ids(c, tokens, n, "tokens"))) return rc;
+  StreamState& s = c->ss[stream];
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  HIP_TRY(c, hipStreamSynchronize(s.st));  // this stream's staging is free again
+  const size_t bt = (size_t)n * 4, bo = (size_t)(ns + 1) * 8;
+  const size_t tb_bytes = compact_temp_bytes(n);
+  if ((rc = ensure_host(c, s.h_tok, bt)) || (rc = ensure_host(c, s.h_off, bo)) ||
+      (rc = ensure_dev(c, s.d_tok, bt)) || (rc = ensure_dev(c, s.d_off, bo)) ||
+      (rc = ensure_dev(c, s.flags, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.pos, (size_t)(n + 1) * 4)) ||
+      (rc = ensure_dev(c, s.kept, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.kept_off, bo)) ||
+      (rc = ensure_dev(c, s.temp, tb_bytes)))
+    return rc;
+  if (bt) memcpy(s.h_tok.p, tokens, bt);
+  memcpy(s.h_off.p, offsets, bo);
+  if (bt) HIP_TRY(c, hipMemcpyAsync(s.d_tok.p, s.h_tok.p, bt, hipMemcpyHostToDevice, s.st));
+  HIP_TRY(c, hipMemcpyAsync(s.d_off.p, s.h_off.p, bo, hipMemcpyHostToDevice, s.st));
+  UpdateArgs a = base_args(c);
+  HIP_TRY(c, launch_compact((const int32_t*)s.d_tok.p, (const int64_t*)s.d_off.p, 0, ns, c->d_subs,
+                            a.mixed_seed, epoch, (int32_t*)s.flags.p, (int32_t*)s.pos.p, (int32_t*)s.kept.p,
+                            (int64_t*)s.kept_off.p, s.temp.p, s.temp.cap, 0, n, first_token, s.st));
+  a.tokens = (const int32_t*)s.kept.p;
+  a.offsets = (const int64_t*)s.kept_off.p;
+  a.neg = nullptr;
+  a.alpha = nullptr;
+  a.unigram = c->d_unigram;
+  a.n_unigram = c->n_unigram;
+  a.epoch = epoch;
+  a.raw_offsets = (const int64_t*)s.d_off.p;
+  a.raw_base = first_token;
+  a.alpha0 = c->alpha0;
+  a.total_words = (uint64_t)c->epochs * c->train_words + 1;
+  a.epoch_words = (uint64_t)epoch * c->train_words;
+  a.n_sent = ns;
   c->words_read += (uint64_t)n;
   return launch_update(c, s, a);
 }

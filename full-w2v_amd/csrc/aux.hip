@@ -21,7 +21,7 @@ hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st) {
 // per-thread LCG. flags[n] = 0 so an exclusive scan of n+1 entries ends in the total.
 __global__ void keep_flags_kernel(const int32_t* __restrict__ corpus, int64_t tok_begin, int64_t n,
                                   const float* __restrict__ subs_p, uint64_t mixed_seed,
-                                  int32_t epoch, int32_t* __restrict__ flags) {
+                                  int32_t epoch, int64_t key_base, int32_t* __restrict__ flags) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   if (i == n) {
@@ -32,7 +32,7 @@ __global__ void keep_flags_kernel(const int32_t* __restrict__ corpus, int64_t to
   if (subs_p) {
     const int64_t pos = tok_begin + i;
     const int32_t w = corpus[pos];
-    const uint64_t x = key2(mixed_seed, pos_key(epoch, pos), 0);
+    const uint64_t x = key2(mixed_seed, pos_key(epoch, key_base + pos), 0);
     keep = !(subs_p[w] < (float)(x & 0xFFFF) / 65536.f);
   }
   flags[i] = keep;
@@ -66,10 +66,10 @@ hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t
                           const float* subs_p, uint64_t mixed_seed, int32_t epoch,
                           int32_t* flags, int32_t* pos, int32_t* kept, int64_t* kept_off,
                           void* temp, size_t temp_bytes, int64_t tok_begin, int64_t n,
-                          hipStream_t st) {
+                          int64_t key_base, hipStream_t st) {
   const int B = 256;
   hipLaunchKernelGGL(keep_flags_kernel, dim3((unsigned)((n + 1 + B - 1) / B)), dim3(B), 0, st,
-                     corpus, tok_begin, n, subs_p, mixed_seed, epoch, flags);
+                     corpus, tok_begin, n, subs_p, mixed_seed, epoch, key_base, flags);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   err = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, pos, (int)(n + 1), st);

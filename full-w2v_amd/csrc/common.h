@@ -103,7 +103,8 @@ struct UpdateArgs {
   int64_t n_unigram;
   uint64_t mixed_seed;
   int32_t epoch;
-  const int64_t* raw_offsets;  // corpus position of each sentence's first token
+  const int64_t* raw_offsets;  // corpus position of each sentence's first token, minus raw_base
+  int64_t raw_base;
   float alpha0;
   uint64_t total_words;  // epochs * train_words + 1 (schedule denominator)
   uint64_t epoch_words;  // epoch * train_words (schedule offset)

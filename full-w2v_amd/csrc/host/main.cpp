@@ -7,9 +7,9 @@
 //    and negatives from their LCG stream (seed = thread id), build GPU sentences of up
 //    to -msl kept tokens and submit batches of -kernel-batch-size sentences round-robin
 //    over -streams library streams;
-//  * device sampling (-device-sampling 1): the corpus is copied to HBM once and each
-//    epoch is trained by w2v_train_device in chunks, with subsampling, negatives and the
-//    learning rate computed on the device.
+//  * device sampling (-device-sampling 1): the corpus is streamed to the GPU in chunks of
+//    whole sentences (pinned staging, two alternating streams) and each chunk is trained
+//    with subsampling, negatives and the learning rate computed on the device.
 #include <pthread.h>
 #include <sched.h>
 #include <stdio.h>
@@ -409,7 +409,7 @@ int main(int argc, char** argv) {
   p.sigmoid = o.exact_sigmoid ? W2V_SIGMOID_EXACT : W2V_SIGMOID_TABLE;
   p.mode = W2V_MODE_HOGWILD;
   p.device = o.device;
-  p.streams = o.device_sampling ? 1 : o.threads * o.streams;
+  p.streams = o.device_sampling ? 2 : o.threads * o.streams;
   p.seed = o.seed;
   w2v_ctx* ctx = nullptr;
   int rc = w2v_create(&p, &ctx);
@@ -449,16 +449,22 @@ int main(int argc, char** argv) {
     if ((rc = w2v_tables_upload(ctx, unigram.data(), kUnigramTableSize, subs.empty() ? nullptr : subs.data())))
       die("w2v_tables_upload", ctx, rc);
     if ((rc = w2v_set_schedule(ctx, o.alpha, o.epochs, vocab.train_words))) die("w2v_set_schedule", ctx, rc);
-    if ((rc = w2v_corpus_upload(ctx, tok.data(), (int64_t)tok.size(), off.data(), n_sent)))
-      die("w2v_corpus_upload", ctx, rc);
+    // chunks of whole sentences, streamed through the library's pinned staging on two
+    // alternating streams so that the copy of one chunk overlaps training of the other
+    const int64_t chunk_tokens = 32 * 1000 * 1000;
+    std::vector<int64_t> rel;
     if ((rc = w2v_sync_all(ctx))) die("w2v_sync_all", ctx, rc);
     t0 = now_s();
-    const int64_t chunk_tokens = 100000000;
+    int turn = 0;
     for (int e = 0; e < o.epochs; e++) {
       for (int64_t sb = 0; sb < n_sent;) {
         int64_t se = sb + 1;
         while (se < n_sent && off[se + 1] - off[sb] <= chunk_tokens) se++;
-        if ((rc = w2v_train_device(ctx, 0, e, sb, se))) die("w2v_train_device", ctx, rc);
+        rel.resize(se - sb + 1);
+        for (int64_t s = sb; s <= se; s++) rel[s - sb] = off[s] - off[sb];
+        if ((rc = w2v_train_chunk(ctx, turn, e, tok.data() + off[sb], rel.data(), se - sb, off[sb])))
+          die("w2v_train_chunk", ctx, rc);
+        turn ^= 1;
         sb = se;
       }
       if (o.debug > 0) {

@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
     if (a.alpha) {
       alpha = a.alpha[s];
     } else {
-      raw = a.raw_offsets[s];
+      raw = a.raw_base + a.raw_offsets[s];
       const uint64_t wca = a.epoch_words + (uint64_t)raw;
       alpha = a.alpha0 * (1.0f - (float)wca / (float)a.total_words);
       if (alpha < a.alpha0 * 0.0001f) alpha = a.alpha0 * 0.0001f;

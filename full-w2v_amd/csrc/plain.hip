@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       alpha = a.alpha[s];
     } else {
       // Train_SGNS learning rate (driver.cu:188-189) at this sentence's corpus position
-      raw = a.raw_offsets[s];
+      raw = a.raw_base + a.raw_offsets[s];
       const uint64_t wca = a.epoch_words + (uint64_t)raw;
       alpha = a.alpha0 * (1.0f - (float)wca / (float)a.total_words);
       if (alpha < a.alpha0 * 0.0001f) alpha = a.alpha0 * 0.0001f;

@@ -131,6 +131,18 @@ int w2v_corpus_upload(w2v_ctx* ctx, const int32_t* tokens, int64_t n_tokens,
 int w2v_train_device(w2v_ctx* ctx, int32_t stream, int32_t epoch, int64_t sent_begin,
                      int64_t sent_end);
 
+/* Streams one chunk of corpus sentences from host memory and trains it with device-side
+ * sampling: sentence s of the chunk holds tokens[offsets[s]..offsets[s+1]) (offsets
+ * relative, offsets[0] = 0), and tokens[0] sits at corpus position first_token (the key of
+ * the sampling streams and the learning-rate position, so a streamed epoch trains exactly
+ * as w2v_train_device over the same corpus). The chunk is copied into the stream's pinned
+ * staging buffer before the call returns, then sent with an asynchronous copy; alternating
+ * two streams double-buffers transfer and training. Needs w2v_tables_upload and
+ * w2v_set_schedule. Replaces the pinned batch buffers and copies of Train_SGNS
+ * (driver.cu:143-154,286-290) without the MSL padding. */
+int w2v_train_chunk(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int32_t* tokens,
+                    const int64_t* offsets, int64_t n_sentences, int64_t first_token);
+
 /* Counters (read after synchronising all streams). */
 int w2v_get_stats(w2v_ctx* ctx, w2v_stats* out);
 

@@ -230,3 +230,39 @@ def test_error_paths(w2vmod):
     with pytest.raises(w2vmod.W2VError):
         tr.sync(stream=3)
     tr.close()
+
+
+@pytest.mark.parametrize("kernel,storage", [("plain", "fp32"), ("mfma", "bf16")])
+def test_streamed_chunks_equal_resident_corpus(orc, w2vmod, kernel, storage):
+    """w2v_train_chunk (host chunks through pinned staging, two streams) trains exactly
+    like w2v_train_device over the same corpus (same keyed draws, same order)."""
+    V, d, W, K, seed = 90, 64, 3, 4, 7
+    rng = np.random.default_rng(21)
+    syn0, syn1 = model(V, d, rng)
+    lens = rng.integers(1, 40, 60)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(1, V, off[-1]).astype(np.int32)
+    cn = np.bincount(tokens, minlength=V).astype(np.uint64) + 1
+    T = int(off[-1])
+    subs = orc.subsample_probs(cn, 0.02, T)
+    uni = orc.unigram_table(cn, 5003)
+    deltas = []
+    for streamed in (False, True):
+        tr = w2vmod.Trainer(V, d, W, K, storage=storage, kernel=kernel, sigmoid="exact",
+                            mode="deterministic", seed=seed, streams=2)
+        tr.upload(syn0, syn1)
+        tr.set_delta_mode(True)
+        tr.tables_upload(uni, subs)
+        tr.set_schedule(0.025, 1, T)
+        if streamed:
+            for i, (a, b) in enumerate([(0, 25), (25, 26), (26, 60)]):
+                tr.train_chunk(0, tokens[off[a]:off[b]], off[a:b + 1] - off[a], int(off[a]), stream=i % 2)
+                tr.sync(i % 2)  # deterministic mode: chunks in order
+        else:
+            tr.corpus_upload(tokens, off)
+            tr.train_device(0, 0, len(lens))
+        deltas.append(tr.delta_download())
+        st = tr.stats()
+        tr.close()
+        assert st["words_read"] == T
+    assert np.array_equal(deltas[0][0], deltas[1][0]) and np.array_equal(deltas[0][1], deltas[1][1])
