@@ -1,0 +1,169 @@
+"""Embedding-quality evaluation of exported vectors.
+
+Restates the reference's evaluation (analysis/evalHyperWords.py driving hyperwords):
+  * word similarity (hyperwords/ws_eval.py:44-49): Spearman correlation between the
+    cosine similarity of each word pair and the human score; a word missing from the
+    vocabulary is represented by a zero vector (similarity 0);
+  * analogies a:a_ :: b:? (hyperwords/analogy_eval.py:48-105): cosine similarities are
+    mapped to (s+1)/2, 3CosAdd = argmax(-s_a + s_a_ + s_b), 3CosMul =
+    argmax(s_a_ * s_b / (s_a + 0.01)), with a, a_ and b themselves excluded.
+Vectors are read from the trainer's export format (text or -binary 1).
+
+Usage: python -m w2v.evaluate vectors.txt [--ws pairs.txt ...] [--analogy questions.txt ...]
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+
+import numpy as np
+
+
+def load_vectors(path):
+    """Reads saveModel output: "V d\\n" then per word "word " + d values + "\\n"."""
+    raw = open(path, "rb").read()
+    head, rest = raw.split(b"\n", 1)
+    V, d = map(int, head.split())
+    words, mat = [], np.zeros((V, d), np.float32)
+    binary = False
+    first = rest.split(b"\n", 1)[0]
+    try:
+        vals = first.split()[1:]
+        binary = len(vals) != d or not all(_isnum(v) for v in vals)
+    except Exception:
+        binary = True
+    pos = 0
+    for i in range(V):
+        sp = rest.index(b" ", pos)
+        words.append(rest[pos:sp].decode("utf-8", "replace"))
+        if binary:
+            mat[i] = np.frombuffer(rest[sp + 1:sp + 1 + 4 * d], np.float32)
+            pos = sp + 1 + 4 * d + 1
+        else:
+            nl = rest.index(b"\n", sp)
+            mat[i] = np.array(rest[sp + 1:nl].split(), np.float32)
+            pos = nl + 1
+    return words, mat
+
+
+def _isnum(b):
+    try:
+        float(b)
+        return True
+    except ValueError:
+        return False
+
+
+class Embedding:
+    """Row-normalised vectors with a word index (hyperwords' Embedding, normalize=True)."""
+
+    def __init__(self, words, mat):
+        self.iw = list(words)
+        self.wi = {w: i for i, w in enumerate(self.iw)}
+        m = np.asarray(mat, np.float64)
+        n = np.linalg.norm(m, axis=1, keepdims=True)
+        n[n == 0] = 1.0
+        self.m = m / n
+
+    def represent(self, w):
+        return self.m[self.wi[w]] if w in self.wi else np.zeros(self.m.shape[1])
+
+    def similarity(self, a, b):
+        return float(self.represent(a) @ self.represent(b))
+
+
+def spearman(x, y):
+    """Spearman rank correlation with average ranks for ties."""
+    def rank(v):
+        v = np.asarray(v, np.float64)
+        order = np.argsort(v, kind="mergesort")
+        r = np.empty(len(v))
+        r[order] = np.arange(len(v))
+        sv = v[order]
+        i = 0
+        while i < len(v):
+            j = i
+            while j + 1 < len(v) and sv[j + 1] == sv[i]:
+                j += 1
+            r[order[i:j + 1]] = (i + j) / 2.0
+            i = j + 1
+        return r
+    rx, ry = rank(x), rank(y)
+    rx -= rx.mean()
+    ry -= ry.mean()
+    den = np.sqrt((rx * rx).sum() * (ry * ry).sum())
+    return float((rx * ry).sum() / den) if den > 0 else 0.0
+
+
+def read_pairs(path):
+    out = []
+    with open(path) as f:
+        for line in f:
+            if line.strip():
+                x, y, s = line.strip().lower().split()
+                out.append((x, y, float(s)))
+    return out
+
+
+def read_analogies(path):
+    out = []
+    with open(path) as f:
+        for line in f:
+            parts = line.strip().lower().split()
+            if len(parts) == 4:
+                out.append(tuple(parts))
+    return out
+
+
+def similarity_eval(emb, pairs):
+    actual = [emb.similarity(x, y) for x, y, _ in pairs]
+    expected = [s for _, _, s in pairs]
+    return spearman(actual, expected)
+
+
+def analogy_eval(emb, questions, batch=256):
+    """Returns (3CosAdd accuracy, 3CosMul accuracy)."""
+    vocab = sorted({w for q in questions for w in q})
+    xi = {w: i for i, w in enumerate(vocab)}
+    vr = np.stack([emb.represent(w) for w in vocab])
+    # hyperwords gives out-of-vocabulary words the representation of the first OOV word
+    # (a zero vector here, as Embedding.represent returns zeros)
+    correct_add = correct_mul = 0
+    for q0 in range(0, len(questions), batch):
+        qs = questions[q0:q0 + batch]
+        ia = np.array([xi[a] for a, _, _, _ in qs])
+        ia_ = np.array([xi[a_] for _, a_, _, _ in qs])
+        ib = np.array([xi[b] for _, _, b, _ in qs])
+        sa = (vr[ia] @ emb.m.T + 1) / 2
+        sa_ = (vr[ia_] @ emb.m.T + 1) / 2
+        sb = (vr[ib] @ emb.m.T + 1) / 2
+        add = -sa + sa_ + sb
+        mul = sa_ * sb / (sa + 0.01)
+        for r, (a, a_, b, b_) in enumerate(qs):
+            for w in (a, a_, b):
+                if w in emb.wi:
+                    add[r, emb.wi[w]] = 0
+                    mul[r, emb.wi[w]] = 0
+            correct_add += emb.iw[int(np.nanargmax(add[r]))] == b_
+            correct_mul += emb.iw[int(np.nanargmax(mul[r]))] == b_
+    n = max(len(questions), 1)
+    return correct_add / n, correct_mul / n
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("vectors")
+    ap.add_argument("--ws", action="append", default=[])
+    ap.add_argument("--analogy", action="append", default=[])
+    args = ap.parse_args(argv)
+    emb = Embedding(*load_vectors(args.vectors))
+    for p in args.ws:
+        print(f"{p}\tspearman\t{similarity_eval(emb, read_pairs(p)):.6f}")
+    for p in args.analogy:
+        add, mul = analogy_eval(emb, read_analogies(p))
+        print(f"{p}\t3CosAdd\t{add:.6f}\t3CosMul\t{mul:.6f}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
