@@ -16,7 +16,7 @@
 namespace w2v {
 hipError_t launch_plain(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st);
 hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st);
-hipError_t occupancy_mfma(int P, int K, int waves_per_block, size_t lds, int* blocks_per_cu);
+hipError_t occupancy_mfma(int P, int K, int W, int waves_per_block, size_t lds, int* blocks_per_cu);
 size_t mfma_lds_per_wave(int W, int K, int P, bool delta);
 bool mfma_supported(int d, int W, int K);
 hipError_t occupancy_plain(int P, int waves_per_block, size_t lds, int* blocks_per_cu);
@@ -274,7 +274,7 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
                 p.kernel, p.storage);
   if (p.kernel == W2V_KERNEL_MFMA && !mfma_supported(p.layer_size, p.window, p.negative))
     return fail(nullptr, W2V_ERR_UNSUPPORTED,
-                "the mfma kernel needs d <= 512, 2W+1 <= 16 and K+1 <= 16 (d=%d W=%d K=%d)",
+                "the mfma kernel needs d <= 512, 2W+1 <= 32 and K+1 <= 16 (d=%d W=%d K=%d)",
                 p.layer_size, p.window, p.negative);
   if (p.sigmoid != W2V_SIGMOID_TABLE && p.sigmoid != W2V_SIGMOID_EXACT)
     return fail(nullptr, W2V_ERR_ARG, "unknown sigmoid mode %d", p.sigmoid);
@@ -335,7 +335,7 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
     int wpb = 4;
     while (wpb > 1 && lw * wpb > lmax) wpb--;
     int bpc = 0;
-    hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, wpb, lw * wpb, &bpc)
+    hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, p.window, wpb, lw * wpb, &bpc)
                        : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
     if (oe != hipSuccess || bpc < 1) bpc = 1;
     c->cfg_wpb[dm] = wpb;

@@ -1,36 +1,7 @@
-// BF16-MFMA SGNS update kernel for gfx950 (window-tile method).
-//
-// For each centre the (contexts x targets) dot products of the whole window are one
-// small dense tile, S = C T^T with C the <= 2W context rows of the LDS ring and T the
-// 1+K target rows, computed on v_mfma_f32_16x16x32_bf16 with FP32 accumulation from one
-// snapshot (a Jacobi step over the window instead of the reference's sequential pair
-// chain, FULL-W2V-kernels.cu:151-215). The sigmoid, the gradient G and both updates
-// dC = G T and dT = G^T C stay FP32 on the vector ALU.
-//
-// Storage is bf16 (half the HBM bytes of the FP32 path). Deltas are added back with the
-// hardware packed-bf16 atomic; stochastic rounding is obtained by dithering each delta
-// with a uniform draw of +-1/2 bf16 ulp of the stored value before the round-to-nearest
-// atomic add, which makes the rounded result x_hi with probability (x+delta-x_lo)/ulp.
-//
-// One wave owns one sentence (persistent grid, device work queue), as in the FP32 kernel.
-// Lane l handles element pairs (2l, 2l+1) + 128e of a row, so row traffic and the packed
-// atomics are coalesced 4-byte-per-lane accesses.
-//
-// Per-wave LDS: ring rows in FP32 (working values), the load-time rows (bf16, or FP32 in
-// delta mode), the bf16 target stage that feeds the MFMA B operand, and the 16x16 G tile.
-#include "common.h"
+// Launch side of the BF16-MFMA window-tile kernel (the kernel is in mfma_impl.h).
+#include "mfma_impl.h"
 
 namespace w2v {
-
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-struct MfmaLayout {
-  int PU;  // FP32 ring row pitch (floats), padded against LDS bank conflicts
-  int PT;  // bf16 target-stage row pitch (elements), padded likewise
-  int off_u0, off_ts, off_gs, wave_bytes;
-};
 
 MfmaLayout mfma_layout(int W, int K, int P, bool delta) {
   MfmaLayout l;
@@ -44,406 +15,9 @@ MfmaLayout mfma_layout(int W, int K, int P, bool delta) {
   l.off_ts = o;
   o = al(o + NT * l.PT * 2);
   l.off_gs = o;
-  o = al(o + 16 * 16 * 4);
+  o = al(o + (R > 16 ? 32 : 16) * 16 * 4);
   l.wave_bytes = o;
   return l;
-}
-
-__device__ __forceinline__ float bf_lo(uint32_t p) { return __uint_as_float(p << 16); }
-__device__ __forceinline__ float bf_hi(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
-__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
-  bf16x2_t v;
-  v[0] = (__bf16)a;  // round to nearest even (v_cvt_pk_bf16_f32)
-  v[1] = (__bf16)b;
-  return __builtin_bit_cast(uint32_t, v);
-}
-
-// +-1/2 bf16 ulp of x, scaled by a 16-bit uniform draw.
-__device__ __forceinline__ float dither(float x, uint32_t u16) {
-  const uint32_t e = (__float_as_uint(x) >> 23) & 0xFFu;
-  if (e <= 7u || e == 0xFFu) return 0.0f;
-  const float ulp = __uint_as_float((e - 7u) << 23);
-  return ((float)u16 * (1.0f / 65536.0f) - 0.5f) * ulp;
-}
-
-__device__ __forceinline__ uint32_t xorshift(uint32_t& s) {
-  s ^= s << 13;
-  s ^= s >> 17;
-  s ^= s << 5;
-  return s;
-}
-
-// Stochastically rounded add of (da, db) to the bf16 pair at p; xa/xb are the stored
-// values the dither is scaled to; va/vb mask elements beyond the row's d columns.
-__device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, float xa, float xb,
-                                            bool va, bool vb, uint32_t& rng) {
-  const uint32_t r = xorshift(rng);
-  da = va ? da + dither(xa, r & 0xFFFFu) : 0.0f;
-  db = vb ? db + dither(xb, r >> 16) : 0.0f;
-  bf16x2_t v;
-  v[0] = (__bf16)da;
-  v[1] = (__bf16)db;
-  __builtin_amdgcn_global_atomic_fadd_v2bf16((bf16x2_t*)p, v);
-}
-
-// Table sigmoid with the reference's buckets (index rule of FULL-W2V-kernels.cu:92-99),
-// the bucket value evaluated in registers with the generator's formula
-// (generate_expTable_h.cpp:48-51: sigma((2i/1000-1)*6), ends 0 and 1) instead of a
-// per-lane table gather; nexpTable[j] equals expTable[999-j], so one formula serves both.
-// Values agree with the stored table to a few ulp (device exp).
-__device__ __forceinline__ float table_value(int i) {
-  if (i <= 0) return 0.0f;
-  if (i >= 999) return 1.0f;
-  const float v = __expf((i / 1000.0f * 2 - 1) * 6);
-  return v / (v + 1);
-}
-
-__device__ __forceinline__ float tile_grad(float f, bool positive, float alpha, const UpdateArgs& a) {
-  if (a.sig_exact) return positive ? alpha / (1.0f + expf(f)) : -alpha / (1.0f + expf(-f));
-  const int idx = sigmoid_index(f);
-  return positive ? alpha * table_value(999 - idx) : -alpha * table_value(idx);
-}
-
-template <int NP, int NTB>
-__global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout ly) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
-  const int g4 = lane >> 4, r16 = lane & 15;
-  const int W = a.W, K = a.K, NT = K + 1, P = a.P, d = a.d;
-  const int R = 2 * W + 1;
-  const bool delta = a.rd0 != nullptr;
-  unsigned char* wb = lds_raw + (size_t)wib * ly.wave_bytes;
-  float* Uf = (float*)wb;
-  unsigned char* U0 = wb + ly.off_u0;
-  uint16_t* Ts = (uint16_t*)(wb + ly.off_ts);
-  float* Gs = (float*)(wb + ly.off_gs);
-  const int cpb = K > 0 ? 64 / K : 0;
-  unsigned long long words = 0, sents = 0;
-
-  for (;;) {
-    unsigned s32 = 0;
-    if (lane == 0) s32 = atomicAdd(a.queue, 1u);
-    const int64_t s = (int64_t)(unsigned)uniform_i((int)s32);
-    if (s >= a.n_sent) break;
-    int64_t base;
-    int L;
-    if (a.offsets) {
-      base = a.offsets[s];
-      L = (int)(a.offsets[s + 1] - base);
-    } else {
-      base = s * (int64_t)a.msl;
-      L = a.length[s];
-    }
-    if (L <= 0) continue;
-    float alpha;
-    int64_t raw = 0;
-    if (a.alpha) {
-      alpha = a.alpha[s];
-    } else {
-      raw = a.raw_base + a.raw_offsets[s];
-      const uint64_t wca = a.epoch_words + (uint64_t)raw;
-      alpha = a.alpha0 * (1.0f - (float)wca / (float)a.total_words);
-      if (alpha < a.alpha0 * 0.0001f) alpha = a.alpha0 * 0.0001f;
-    }
-    words += (unsigned long long)L;
-    sents += 1;
-    uint32_t rng = (uint32_t)key2(a.mixed_seed, 0x5352ull ^ ((uint64_t)a.epoch << 40), (uint64_t)(s * 64 + lane)) | 1u;
-
-    int chunk0 = 0;
-    int tokbuf = (lane < L) ? a.tokens[base + lane] : 0;
-    int ring_tok = 0;
-    int negbuf = 0, nw0 = -(1 << 30);
-
-    auto load_row = [&](int slot, int tok) {
-      const uint32_t* src = (const uint32_t*)(a.bsyn0 + (size_t)tok * P);
-      float* uf = Uf + slot * ly.PU;
-#pragma unroll
-      for (int e = 0; e < NP; e++) {
-        const int j = 2 * lane + 128 * e;
-        if (j < P) {
-          const uint32_t pr = src[j >> 1];
-          float x0 = bf_lo(pr), x1 = bf_hi(pr);
-          if (delta) {
-            x0 += a.rd0[(size_t)tok * P + j];
-            x1 += a.rd0[(size_t)tok * P + j + 1];
-            *(float2*)((float*)U0 + slot * P + j) = make_float2(x0, x1);
-          } else {
-            ((uint32_t*)U0)[(slot * P + j) >> 1] = pr;
-          }
-          *(float2*)(uf + j) = make_float2(x0, x1);
-        }
-      }
-    };
-    auto flush_row = [&](int slot, int tok) {
-      const float* uf = Uf + slot * ly.PU;
-#pragma unroll
-      for (int e = 0; e < NP; e++) {
-        const int j = 2 * lane + 128 * e;
-        if (j < d) {
-          const float2 u = *(const float2*)(uf + j);
-          float x0, x1;
-          if (delta) {
-            const float2 v = *(const float2*)((const float*)U0 + slot * P + j);
-            x0 = v.x;
-            x1 = v.y;
-            atomicAdd(a.out0 + (size_t)tok * P + j, u.x - x0);
-            if (j + 1 < d) atomicAdd(a.out0 + (size_t)tok * P + j + 1, u.y - x1);
-          } else {
-            const uint32_t pr = ((const uint32_t*)U0)[(slot * P + j) >> 1];
-            x0 = bf_lo(pr);
-            x1 = bf_hi(pr);
-            sr_add_pair(a.bsyn0 + (size_t)tok * P + j, u.x - x0, u.y - x1, x0, x1, true, j + 1 < d, rng);
-          }
-        }
-      }
-    };
-    auto negative = [&](int w, int k) -> int {
-      if (w >= nw0 + cpb) {
-        nw0 = w;
-        const int cw = nw0 + lane / K, kk = lane % K;
-        int v = 0;
-        if (lane < cpb * K && cw < L) {
-          if (a.neg) {
-            v = a.neg[(base + cw) * K + kk];
-          } else {
-            const uint64_t x = key2(a.mixed_seed, pos_key(a.epoch, raw + cw), (uint64_t)kk + 1);
-            v = a.unigram[((x >> 32) * (uint64_t)a.n_unigram) >> 32];
-          }
-        }
-        negbuf = v;
-      }
-      return readlane_i(negbuf, (w - nw0) * K + k);
-    };
-
-    const int npre = L < W + 1 ? L : W + 1;
-    for (int p = 0; p < npre; p++) {
-      const int tok = readlane_i(tokbuf, p);
-      ring_tok = (lane == p) ? tok : ring_tok;
-      load_row(p, tok);
-    }
-    // Hogwild mode loads the next centre's target rows and the next entering context
-    // row one centre ahead (packed bf16 pairs); deterministic mode loads in order.
-    const bool pipe = !a.fence && !delta;
-    auto entering_token = [&](int pos) -> int {
-      if (pos - chunk0 >= 64) {
-        chunk0 += 64;
-        tokbuf = (chunk0 + lane < L) ? a.tokens[base + chunk0 + lane] : 0;
-      }
-      return readlane_i(tokbuf, pos - chunk0);
-    };
-    int ntk[NTB];
-    uint32_t nT[NTB][NP], nE[NP];
-    auto targets_of = [&](int w, int slot_of_w, int* ids) {
-#pragma unroll
-      for (int k = 0; k < NTB; k++) {
-        ids[k] = 0;
-        if (k < NT) ids[k] = (k == 0) ? readlane_i(ring_tok, slot_of_w) : negative(w, k - 1);
-      }
-    };
-    auto fetch_pairs = [&](const uint16_t* m, int row, uint32_t* dst) {
-#pragma unroll
-      for (int e = 0; e < NP; e++) {
-        const int j = 2 * lane + 128 * e;
-        dst[e] = (j < P) ? ((const uint32_t*)(m + (size_t)row * P))[j >> 1] : 0u;
-      }
-    };
-    if (pipe) {
-      targets_of(0, 0, ntk);
-#pragma unroll
-      for (int k = 0; k < NTB; k++)
-        if (k < NT) fetch_pairs(a.bsyn1, ntk[k], nT[k]);
-    }
-
-    int cs = 0;
-    for (int w = 0; w < L; w++) {
-      const int enter_pos = w + W + 1;
-      int enter_tok = 0;
-      if (enter_pos < L) {
-        enter_tok = entering_token(enter_pos);
-        if (pipe) fetch_pairs(a.bsyn0, enter_tok, nE);
-      }
-      // ---- targets: fresh rows, FP32 in registers and bf16 in the MFMA stage
-      int tk[NTB];
-      float Tf[NTB][NP][2];
-      uint32_t cT[NTB][NP];
-      if (pipe) {
-#pragma unroll
-        for (int k = 0; k < NTB; k++) {
-          tk[k] = ntk[k];
-#pragma unroll
-          for (int e = 0; e < NP; e++) cT[k][e] = nT[k][e];
-        }
-        if (w + 1 < L) {  // next centre's targets
-          targets_of(w + 1, cs + 1 == R ? 0 : cs + 1, ntk);
-#pragma unroll
-          for (int k = 0; k < NTB; k++)
-            if (k < NT) fetch_pairs(a.bsyn1, ntk[k], nT[k]);
-        }
-      } else {
-        targets_of(w, cs, tk);
-#pragma unroll
-        for (int k = 0; k < NTB; k++)
-          if (k < NT) fetch_pairs(a.bsyn1, tk[k], cT[k]);
-      }
-#pragma unroll
-      for (int k = 0; k < NTB; k++) {
-#pragma unroll
-        for (int e = 0; e < NP; e++) {
-          const int j = 2 * lane + 128 * e;
-          float x0 = 0.0f, x1 = 0.0f;
-          if (k < NT && j < P) {
-            const uint32_t pr = cT[k][e];
-            x0 = bf_lo(pr);
-            x1 = bf_hi(pr);
-            if (delta) {
-              x0 += a.rd1[(size_t)tk[k] * P + j];
-              x1 += a.rd1[(size_t)tk[k] * P + j + 1];
-            }
-            ((uint32_t*)Ts)[(k * ly.PT + j) >> 1] = delta ? pack_bf16(x0, x1) : pr;
-          }
-          Tf[k][e][0] = x0;
-          Tf[k][e][1] = x1;
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-
-      // ---- S = C T^T on MFMA (rows: ring slots, columns: targets)
-      f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int s4 = 0; s4 < P / 32; s4++) {
-        bf16x8_t A, B;
-        const int j0 = 32 * s4 + 8 * g4;
-        if (r16 < R) {
-          const float4 lo = *(const float4*)(Uf + r16 * ly.PU + j0);
-          const float4 hi = *(const float4*)(Uf + r16 * ly.PU + j0 + 4);
-          A[0] = (__bf16)lo.x; A[1] = (__bf16)lo.y; A[2] = (__bf16)lo.z; A[3] = (__bf16)lo.w;
-          A[4] = (__bf16)hi.x; A[5] = (__bf16)hi.y; A[6] = (__bf16)hi.z; A[7] = (__bf16)hi.w;
-        } else {
-#pragma unroll
-          for (int i = 0; i < 8; i++) A[i] = (__bf16)0.0f;
-        }
-        if (r16 < NT) {
-          B = *(const bf16x8_t*)(Ts + r16 * ly.PT + j0);
-        } else {
-#pragma unroll
-          for (int i = 0; i < 8; i++) B[i] = (__bf16)0.0f;
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
-      }
-
-      // ---- G = g(S) on the valid (context, target) cells; lane holds S[4*g4+i][r16]
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int c = 4 * g4 + i;
-        float gv = 0.0f;
-        if (r16 < NT && c < R) {
-          int rel = c - cs;
-          if (rel < 0) rel += R;
-          if (rel > W) rel -= R;
-          const int p = w + rel;
-          if (rel != 0 && p >= 0 && p < L) gv = tile_grad(acc[i], r16 == 0, alpha, a);
-        }
-        Gs[c * 16 + r16] = gv;
-      }
-      __builtin_amdgcn_wave_barrier();
-
-      // ---- dC = G T (applied to the ring) and dT = G^T C (old C), FP32
-      float dT[NTB][NP][2];
-#pragma unroll
-      for (int k = 0; k < NTB; k++)
-#pragma unroll
-        for (int e = 0; e < NP; e++) dT[k][e][0] = dT[k][e][1] = 0.0f;
-      for (int rel = -W; rel <= W; rel++) {
-        const int p = w + rel;
-        if (rel == 0 || p < 0 || p >= L) continue;
-        int c = cs + rel;
-        if (c < 0) c += R;
-        if (c >= R) c -= R;
-        float gk[NTB];
-#pragma unroll
-        for (int k = 0; k < NTB; k++) gk[k] = Gs[c * 16 + k];
-        float* uf = Uf + c * ly.PU;
-#pragma unroll
-        for (int e = 0; e < NP; e++) {
-          const int j = 2 * lane + 128 * e;
-          if (j < P) {
-            const float2 u = *(const float2*)(uf + j);
-            float du0 = 0.0f, du1 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < NTB; k++) {
-              du0 = fmaf(gk[k], Tf[k][e][0], du0);
-              du1 = fmaf(gk[k], Tf[k][e][1], du1);
-              dT[k][e][0] = fmaf(gk[k], u.x, dT[k][e][0]);
-              dT[k][e][1] = fmaf(gk[k], u.y, dT[k][e][1]);
-            }
-            *(float2*)(uf + j) = make_float2(u.x + du0, u.y + du1);
-          }
-        }
-      }
-
-      // ---- target deltas back to the model
-#pragma unroll
-      for (int k = 0; k < NTB; k++) {
-        if (k < NT) {
-#pragma unroll
-          for (int e = 0; e < NP; e++) {
-            const int j = 2 * lane + 128 * e;
-            if (j < d) {
-              if (delta) {
-                atomicAdd(a.out1 + (size_t)tk[k] * P + j, dT[k][e][0]);
-                if (j + 1 < d) atomicAdd(a.out1 + (size_t)tk[k] * P + j + 1, dT[k][e][1]);
-              } else {
-                sr_add_pair(a.bsyn1 + (size_t)tk[k] * P + j, dT[k][e][0], dT[k][e][1], Tf[k][e][0],
-                            Tf[k][e][1], true, j + 1 < d, rng);
-              }
-            }
-          }
-        }
-      }
-      if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-
-      // ---- ring advance: flush the leaving context, load the entering one
-      const int leave = w - W, enter = w + W + 1;
-      int slot = cs + W + 1;
-      if (slot >= R) slot -= R;
-      if (leave >= 0) {
-        flush_row(slot, readlane_i(ring_tok, slot));
-        if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-      }
-      if (enter < L) {
-        ring_tok = (lane == slot) ? enter_tok : ring_tok;
-        if (pipe) {
-          float* uf = Uf + slot * ly.PU;
-#pragma unroll
-          for (int e = 0; e < NP; e++) {
-            const int j = 2 * lane + 128 * e;
-            if (j < P) {
-              ((uint32_t*)U0)[(slot * P + j) >> 1] = nE[e];
-              *(float2*)(uf + j) = make_float2(bf_lo(nE[e]), bf_hi(nE[e]));
-            }
-          }
-        } else {
-          load_row(slot, enter_tok);
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      cs = (cs + 1 == R) ? 0 : cs + 1;
-    }
-    for (int p = (L - W > 0 ? L - W : 0); p < L; p++) {
-      const int slot = p % R;
-      flush_row(slot, readlane_i(ring_tok, slot));
-    }
-    if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (lane == 0 && a.counters) {
-    atomicAdd(a.counters + 0, words);
-    atomicAdd(a.counters + 1, sents);
-  }
-}
-
-template <int NP, int NTB>
-static const void* mfma_fn() {
-  return (const void*)sgns_mfma_kernel<NP, NTB>;
 }
 
 static int nt_bucket(int nt) {
@@ -455,24 +29,21 @@ static int nt_bucket(int nt) {
   return 16;
 }
 
-#define W2V_NTB_CASES(NP, M) M(NP, 2) M(NP, 4) M(NP, 6) M(NP, 8) M(NP, 12) M(NP, 16)
-
-static const void* mfma_select(int P, int NT) {
-  const int np = (P + 127) / 128, ntb = nt_bucket(NT);
-#define CASE(NP, NTB) \
-  if (np == NP && ntb == NTB) return mfma_fn<NP, NTB>();
-  W2V_NTB_CASES(1, CASE)
-  W2V_NTB_CASES(2, CASE)
-  W2V_NTB_CASES(3, CASE)
-  W2V_NTB_CASES(4, CASE)
-#undef CASE
-  return nullptr;
+static const void* mfma_select(int P, int NT, int R) {
+  const int np = (P + 127) / 128, ntb = nt_bucket(NT), mt = R > 16 ? 2 : 1;
+  switch (np) {
+    case 1: return mfma_instance_np1(ntb, mt);
+    case 2: return mfma_instance_np2(ntb, mt);
+    case 3: return mfma_instance_np3(ntb, mt);
+    case 4: return mfma_instance_np4(ntb, mt);
+    default: return nullptr;
+  }
 }
 
-bool mfma_supported(int d, int W, int K) { return d <= 512 && 2 * W + 1 <= 16 && K + 1 <= 16; }
+bool mfma_supported(int d, int W, int K) { return d <= 512 && 2 * W + 1 <= 32 && K + 1 <= 16; }
 
 hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st) {
-  const void* fn = mfma_select(a.P, a.K + 1);
+  const void* fn = mfma_select(a.P, a.K + 1, 2 * a.W + 1);
   if (!fn) return hipErrorInvalidValue;
   const MfmaLayout ly = mfma_layout(a.W, a.K, a.P, a.rd0 != nullptr);
   const size_t lds = (size_t)ly.wave_bytes * waves_per_block;
@@ -484,8 +55,8 @@ hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hip
   return hipLaunchKernel(fn, dim3(blocks), dim3(64 * waves_per_block), args, lds, st);
 }
 
-hipError_t occupancy_mfma(int P, int K, int waves_per_block, size_t lds, int* blocks_per_cu) {
-  const void* fn = mfma_select(P, K + 1);
+hipError_t occupancy_mfma(int P, int K, int W, int waves_per_block, size_t lds, int* blocks_per_cu) {
+  const void* fn = mfma_select(P, K + 1, 2 * W + 1);
   if (!fn) return hipErrorInvalidValue;
   hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;

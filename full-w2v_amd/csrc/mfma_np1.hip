@@ -1,0 +1,15 @@
+// Instantiations of the BF16-MFMA kernel for rows of up to 128 elements.
+#include "mfma_impl.h"
+
+namespace w2v {
+
+const void* mfma_instance_np1(int ntb, int mt) {
+#define W2V_NT(NTB)                                                          \
+  if (ntb == NTB) return mt == 1 ? (const void*)sgns_mfma_kernel<1, NTB, 1> \
+                                 : (const void*)sgns_mfma_kernel<1, NTB, 2>;
+  W2V_NT(2) W2V_NT(4) W2V_NT(6) W2V_NT(8) W2V_NT(12) W2V_NT(16)
+#undef W2V_NT
+  return nullptr;
+}
+
+}  // namespace w2v
