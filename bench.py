@@ -181,6 +181,33 @@ def cpu_baseline(host, target_s=15.0):
                        f"FP32 Hogwild, table sigmoid")
 
 
+def copy_calibration():
+    """A plain copy kernel of known size (1 GiB read, 1 GiB written, five launches) used
+    to scale the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    import torch
+
+    n = (1 << 30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the update kernel from the committed rocprofv3 --pmc
+    summary (profiles/traffic_<kernel>.md, line 'bytes_per_launch: <n>'), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.md")
+    try:
+        for line in open(path):
+            if line.startswith("bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,7 +215,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="all", help="all | plain | mfma")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only the copy kernel used to scale HBM byte counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -230,7 +262,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": best["achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(best["kernel"]),
         },
         "cpu_baseline": cpu,
         "variants": results,

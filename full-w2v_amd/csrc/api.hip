@@ -79,6 +79,8 @@ struct w2v_ctx {
   uint16_t* bsyn1 = nullptr;
   int cfg_wpb[2] = {0, 0};  // update-kernel launch shape, [delta mode]
   int cfg_blocks[2] = {0, 0};
+  int max_blocks[2] = {0, 0};  // device-filling grid at cfg_wpb
+  int user_wpb = 0, user_blocks = 0;  // w2v_set_launch_shape override
   float* dsyn0 = nullptr;
   float* dsyn1 = nullptr;
   int delta_mode = 0;
@@ -225,6 +227,11 @@ int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a) {
   HIP_TRY(c, launch_zero_u32(s.queue, 1, s.st));
   const int dm = c->delta_mode ? 1 : 0;
   int wpb = c->cfg_wpb[dm], blocks = c->cfg_blocks[dm];
+  if (c->user_wpb > 0) {
+    wpb = c->user_wpb < wpb ? c->user_wpb : wpb;  // never more waves than the LDS allows
+    blocks = c->max_blocks[dm] * c->cfg_wpb[dm] / wpb;
+  }
+  if (c->user_blocks > 0 && c->user_blocks < blocks) blocks = c->user_blocks;
   if (c->p.mode == W2V_MODE_DETERMINISTIC) {
     wpb = 1;
     blocks = 1;
@@ -340,6 +347,7 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
     if (oe != hipSuccess || bpc < 1) bpc = 1;
     c->cfg_wpb[dm] = wpb;
     c->cfg_blocks[dm] = bpc * c->num_cu;
+    c->max_blocks[dm] = bpc * c->num_cu;
   }
   c->ss.resize(p.streams);
   for (auto& s : c->ss) {
@@ -724,6 +732,27 @@ int w2v_train_chunk(w2v_ctx* c, int32_t stream, int32_t epoch, const int32_t* to
   a.n_sent = ns;
   c->words_read += (uint64_t)n;
   return launch_update(c, s, a);
+}
+
+int w2v_set_launch_shape(w2v_ctx* c, int32_t wpb, int32_t blocks) {
+  if (!c || wpb < 0 || wpb > 4 || blocks < 0) return W2V_ERR_ARG;
+  c->user_wpb = wpb;
+  c->user_blocks = blocks;
+  return W2V_OK;
+}
+
+int w2v_get_launch_shape(w2v_ctx* c, int32_t* wpb, int32_t* blocks) {
+  if (!c) return W2V_ERR_ARG;
+  const int dm = c->delta_mode ? 1 : 0;
+  int w = c->cfg_wpb[dm], b = c->cfg_blocks[dm];
+  if (c->user_wpb > 0) {
+    w = c->user_wpb < w ? c->user_wpb : w;
+    b = c->max_blocks[dm] * c->cfg_wpb[dm] / w;
+  }
+  if (c->user_blocks > 0 && c->user_blocks < b) b = c->user_blocks;
+  if (wpb) *wpb = w;
+  if (blocks) *blocks = b;
+  return W2V_OK;
 }
 
 int w2v_get_stats(w2v_ctx* c, w2v_stats* out) {

@@ -29,7 +29,7 @@ EXPORTS = [
     "w2v_corpus_upload", "w2v_train_device", "w2v_train_chunk", "w2v_get_stats", "w2v_heldout_loss",
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
     "w2v_synth_corpus", "w2v_synth_cluster_of", "w2v_corpus_info", "w2v_corpus_download",
-    "w2v_corpus_counts",
+    "w2v_corpus_counts", "w2v_set_launch_shape", "w2v_get_launch_shape",
 ]
 
 
@@ -97,6 +97,8 @@ def lib() -> ctypes.CDLL:
             "w2v_corpus_info": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
             "w2v_corpus_download": [vp, i64, i64, vp, vp],
             "w2v_corpus_counts": [vp, vp],
+            "w2v_set_launch_shape": [vp, i32, i32],
+            "w2v_get_launch_shape": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -282,6 +284,14 @@ class Trainer:
         c = np.zeros(self.V, np.uint64)
         self._chk(lib().w2v_corpus_counts(self.h, _ptr(c)))
         return c
+
+    def set_launch_shape(self, waves_per_block=0, blocks=0):
+        self._chk(lib().w2v_set_launch_shape(self.h, int(waves_per_block), int(blocks)))
+
+    def launch_shape(self):
+        w, b = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(lib().w2v_get_launch_shape(self.h, ctypes.byref(w), ctypes.byref(b)))
+        return w.value, b.value
 
     def kernel_time(self, stream=0):
         ms = ctypes.c_double()

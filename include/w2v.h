@@ -143,6 +143,14 @@ int w2v_train_device(w2v_ctx* ctx, int32_t stream, int32_t epoch, int64_t sent_b
 int w2v_train_chunk(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int32_t* tokens,
                     const int64_t* offsets, int64_t n_sentences, int64_t first_token);
 
+/* Launch shape of the update kernel in Hogwild mode: waves per workgroup (1..4) and
+ * persistent workgroups (0 = as many as fit on the device). Bounds how many sentences
+ * train concurrently. (0, 0) restores the default. No counterpart in the reference,
+ * whose shape is fixed by -kernel-batch-size and d (kernels.cu:15-32). */
+int w2v_set_launch_shape(w2v_ctx* ctx, int32_t waves_per_block, int32_t blocks);
+/* Current launch shape (after defaults are resolved). */
+int w2v_get_launch_shape(w2v_ctx* ctx, int32_t* waves_per_block, int32_t* blocks);
+
 /* Counters (read after synchronising all streams). */
 int w2v_get_stats(w2v_ctx* ctx, w2v_stats* out);
 

@@ -266,3 +266,37 @@ def test_streamed_chunks_equal_resident_corpus(orc, w2vmod, kernel, storage):
         tr.close()
         assert st["words_read"] == T
     assert np.array_equal(deltas[0][0], deltas[1][0]) and np.array_equal(deltas[0][1], deltas[1][1])
+
+
+def distinct_batch(V, lens, K, rng):
+    """Tokens all distinct and negatives all distinct and disjoint from them, so no row is
+    read again after it is written (prefetching cannot observe staleness)."""
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    perm = rng.permutation(np.arange(1, V))
+    tokens = perm[:off[-1]].astype(np.int32)
+    neg = perm[off[-1]:off[-1] + off[-1] * K].astype(np.int32)
+    assert len(neg) == off[-1] * K
+    return tokens, off, neg
+
+
+def test_hogwild_path_one_wave_matches_oracle(orc, w2vmod):
+    """Hogwild mode (no fences, rows loaded one step ahead) on a single wave: with no row
+    revisited it must equal the serialised FP64 update to the deterministic tolerance."""
+    V, d, W, K = 4000, 128, 5, 5
+    rng = np.random.default_rng(31)
+    syn0, syn1 = model(V, d, rng)
+    lens = [20, 1, 7, 40, 12]
+    tokens, off, neg = distinct_batch(V, lens, K, rng)
+    alpha = np.full(len(lens), 0.025, np.float32)
+    tr = w2vmod.Trainer(V, d, W, K, msl=64, sigmoid="exact", mode="hogwild")
+    tr.set_launch_shape(1, 1)
+    assert tr.launch_shape() == (1, 1)
+    tr.upload(syn0, syn1)
+    tr.set_delta_mode(True)
+    tr.submit_csr(tokens, off, neg, alpha)
+    g0, g1 = tr.delta_download()
+    tr.close()
+    r0, r1 = oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha)
+    e0, e1 = row_rel_err(g0, r0), row_rel_err(g1, r1)
+    print(f"hogwild path, 1 wave: syn0 rel {e0:.2e} syn1 rel {e1:.2e}", flush=True)
+    assert e0 <= TOL and e1 <= TOL
