@@ -269,8 +269,9 @@ def main():
     }
     if "plain" in results and "mfma" in results:
         line["speedup_mfma_vs_plain"] = results["mfma"]["words_per_sec"] / results["plain"]["words_per_sec"]
-        line["heldout_loss_rel_diff"] = (results["mfma"]["heldout_loss"] - results["plain"]["heldout_loss"]) \
-            / results["plain"]["heldout_loss"]
+        if results["plain"]["heldout_loss"] > 0:
+            line["heldout_loss_rel_diff"] = (results["mfma"]["heldout_loss"] - results["plain"]["heldout_loss"]) \
+                / results["plain"]["heldout_loss"]
     if cpu:
         line["gpu_vs_cpu"] = best["words_per_sec"] / cpu["value"]
     print(json.dumps(line), flush=True)

@@ -339,15 +339,23 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
     const size_t lmax = 160 * 1024;
     if (lw > lmax)
       return bail(fail(c, W2V_ERR_UNSUPPORTED, "per-wave LDS of %zu bytes exceeds 160 KiB", lw));
-    int wpb = 4;
-    while (wpb > 1 && lw * wpb > lmax) wpb--;
-    int bpc = 0;
-    hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, p.window, wpb, lw * wpb, &bpc)
-                       : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
-    if (oe != hipSuccess || bpc < 1) bpc = 1;
-    c->cfg_wpb[dm] = wpb;
-    c->cfg_blocks[dm] = bpc * c->num_cu;
-    c->max_blocks[dm] = bpc * c->num_cu;
+    // the shape with the most resident waves per CU (LDS, registers), larger blocks on ties
+    int best_wpb = 1, best_bpc = 1, best_waves = 0;
+    for (int wpb = 4; wpb >= 1; wpb--) {
+      if (lw * wpb > lmax) continue;
+      int bpc = 0;
+      hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, p.window, wpb, lw * wpb, &bpc)
+                         : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
+      if (oe != hipSuccess || bpc < 1) bpc = 1;
+      if (bpc * wpb > best_waves) {
+        best_waves = bpc * wpb;
+        best_wpb = wpb;
+        best_bpc = bpc;
+      }
+    }
+    c->cfg_wpb[dm] = best_wpb;
+    c->cfg_blocks[dm] = best_bpc * c->num_cu;
+    c->max_blocks[dm] = best_bpc * c->num_cu;
   }
   c->ss.resize(p.streams);
   for (auto& s : c->ss) {

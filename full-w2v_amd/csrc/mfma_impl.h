@@ -44,14 +44,6 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
   return __builtin_bit_cast(uint32_t, v);
 }
 
-// +-1/2 bf16 ulp of x, scaled by a 16-bit uniform draw.
-__device__ __forceinline__ float dither(float x, uint32_t u16) {
-  const uint32_t e = (__float_as_uint(x) >> 23) & 0xFFu;
-  if (e <= 7u || e == 0xFFu) return 0.0f;
-  const float ulp = __uint_as_float((e - 7u) << 23);
-  return ((float)u16 * (1.0f / 65536.0f) - 0.5f) * ulp;
-}
-
 __device__ __forceinline__ uint32_t xorshift(uint32_t& s) {
   s ^= s << 13;
   s ^= s >> 17;
@@ -59,13 +51,23 @@ __device__ __forceinline__ uint32_t xorshift(uint32_t& s) {
   return s;
 }
 
-// Stochastically rounded add of (da, db) to the bf16 pair at p; xa/xb are the stored
-// values the dither is scaled to; va/vb mask elements beyond the row's d columns.
-__device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, float xa, float xb,
-                                            bool va, bool vb, uint32_t& rng) {
+// Stochastically rounded add of (da, db) to the bf16 pair at p. xp is the pair as last
+// read (two bf16 in one word): the dither of each element is u * ulp with u uniform in
+// [-1/2, 1/2) and ulp = 2^-7 of that element's binade, taken from the bf16 exponent bits
+// with one packed 16-bit subtraction. vb masks the second element (odd d).
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, uint32_t xp, bool vb,
+                                            uint32_t& rng) {
   const uint32_t r = xorshift(rng);
-  da = va ? da + dither(xa, r & 0xFFFFu) : 0.0f;
-  db = vb ? db + dither(xb, r >> 16) : 0.0f;
+  u16x2_t ex = __builtin_bit_cast(u16x2_t, xp & 0x7F807F80u);
+  const u16x2_t seven = {(unsigned short)(7u << 7), (unsigned short)(7u << 7)};
+  ex = ex > seven ? ex - seven : u16x2_t{0, 0};  // 2^(e-7) per half, 0 for tiny values
+  const uint32_t ul = __builtin_bit_cast(uint32_t, ex);
+  const float ua = __uint_as_float(0x3F800000u | ((r >> 16) << 7)) - 1.5f;  // 16 bits each
+  const float ub = __uint_as_float(0x3F800000u | ((r & 0xFFFFu) << 7)) - 1.5f;
+  da = fmaf(ua, __uint_as_float(ul << 16), da);
+  db = vb ? fmaf(ub, __uint_as_float(ul & 0xFFFF0000u), db) : 0.0f;
   bf16x2_t v;
   v[0] = (__bf16)da;
   v[1] = (__bf16)db;
@@ -178,7 +180,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
             const uint32_t pr = ((const uint32_t*)U0)[(slot * P + j) >> 1];
             x0 = bf_lo(pr);
             x1 = bf_hi(pr);
-            sr_add_pair(a.bsyn0 + (size_t)tok * P + j, u.x - x0, u.y - x1, x0, x1, true, j + 1 < d, rng);
+            sr_add_pair(a.bsyn0 + (size_t)tok * P + j, u.x - x0, u.y - x1, pr, j + 1 < d, rng);
           }
         }
       }
@@ -388,8 +390,8 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
                 atomicAdd(a.out1 + (size_t)tk[k] * P + j, dT[k][e][0]);
                 if (j + 1 < d) atomicAdd(a.out1 + (size_t)tk[k] * P + j + 1, dT[k][e][1]);
               } else {
-                sr_add_pair(a.bsyn1 + (size_t)tk[k] * P + j, dT[k][e][0], dT[k][e][1], Tf[k][e][0],
-                            Tf[k][e][1], true, j + 1 < d, rng);
+                sr_add_pair(a.bsyn1 + (size_t)tk[k] * P + j, dT[k][e][0], dT[k][e][1], cT[k][e],
+                            j + 1 < d, rng);
               }
             }
           }
