@@ -16,7 +16,8 @@
 namespace w2v {
 hipError_t launch_plain(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st);
 hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st);
-hipError_t occupancy_mfma(int P, int K, int W, int waves_per_block, size_t lds, int* blocks_per_cu);
+hipError_t occupancy_mfma(int P, int K, int W, bool delta, int waves_per_block, size_t lds,
+                          int* blocks_per_cu);
 size_t mfma_lds_per_wave(int W, int K, int P, bool delta);
 bool mfma_supported(int d, int W, int K);
 hipError_t occupancy_plain(int P, int waves_per_block, size_t lds, int* blocks_per_cu);
@@ -344,7 +345,7 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
     for (int wpb = 4; wpb >= 1; wpb--) {
       if (lw * wpb > lmax) continue;
       int bpc = 0;
-      hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, p.window, wpb, lw * wpb, &bpc)
+      hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, p.window, dm != 0, wpb, lw * wpb, &bpc)
                          : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
       if (oe != hipSuccess || bpc < 1) bpc = 1;
       if (bpc * wpb > best_waves) {

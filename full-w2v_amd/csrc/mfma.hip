@@ -29,13 +29,13 @@ static int nt_bucket(int nt) {
   return 16;
 }
 
-static const void* mfma_select(int P, int NT, int R) {
+static const void* mfma_select(int P, int NT, int R, bool hw) {
   const int np = (P + 127) / 128, ntb = nt_bucket(NT), mt = R > 16 ? 2 : 1;
   switch (np) {
-    case 1: return mfma_instance_np1(ntb, mt);
-    case 2: return mfma_instance_np2(ntb, mt);
-    case 3: return mfma_instance_np3(ntb, mt);
-    case 4: return mfma_instance_np4(ntb, mt);
+    case 1: return mfma_instance_np1(ntb, mt, hw);
+    case 2: return mfma_instance_np2(ntb, mt, hw);
+    case 3: return mfma_instance_np3(ntb, mt, hw);
+    case 4: return mfma_instance_np4(ntb, mt, hw);
     default: return nullptr;
   }
 }
@@ -43,7 +43,7 @@ static const void* mfma_select(int P, int NT, int R) {
 bool mfma_supported(int d, int W, int K) { return d <= 512 && 2 * W + 1 <= 32 && K + 1 <= 16; }
 
 hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st) {
-  const void* fn = mfma_select(a.P, a.K + 1, 2 * a.W + 1);
+  const void* fn = mfma_select(a.P, a.K + 1, 2 * a.W + 1, a.rd0 == nullptr && !a.fence);
   if (!fn) return hipErrorInvalidValue;
   const MfmaLayout ly = mfma_layout(a.W, a.K, a.P, a.rd0 != nullptr);
   const size_t lds = (size_t)ly.wave_bytes * waves_per_block;
@@ -55,8 +55,9 @@ hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hip
   return hipLaunchKernel(fn, dim3(blocks), dim3(64 * waves_per_block), args, lds, st);
 }
 
-hipError_t occupancy_mfma(int P, int K, int W, int waves_per_block, size_t lds, int* blocks_per_cu) {
-  const void* fn = mfma_select(P, K + 1, 2 * W + 1);
+hipError_t occupancy_mfma(int P, int K, int W, bool delta, int waves_per_block, size_t lds,
+                          int* blocks_per_cu) {
+  const void* fn = mfma_select(P, K + 1, 2 * W + 1, !delta);
   if (!fn) return hipErrorInvalidValue;
   hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;

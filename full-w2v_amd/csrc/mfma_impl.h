@@ -78,12 +78,12 @@ __device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, uin
 // the bucket value evaluated in registers with the generator's formula
 // (generate_expTable_h.cpp:48-51: sigma((2i/1000-1)*6), ends 0 and 1) instead of a
 // per-lane table gather; nexpTable[j] equals expTable[999-j], so one formula serves both.
-// Values agree with the stored table to a few ulp (device exp).
+// Values agree with the stored table to a few ulp (device exp and reciprocal).
 __device__ __forceinline__ float table_value(int i) {
   if (i <= 0) return 0.0f;
   if (i >= 999) return 1.0f;
-  const float v = __expf((i / 1000.0f * 2 - 1) * 6);
-  return v / (v + 1);
+  const float v = __expf((1 - i / 1000.0f * 2) * 6);  // e^-x
+  return __builtin_amdgcn_rcpf(1.0f + v);           // e^x / (e^x + 1)
 }
 
 __device__ __forceinline__ float tile_grad(float f, bool positive, float alpha, const UpdateArgs& a) {
@@ -92,14 +92,21 @@ __device__ __forceinline__ float tile_grad(float f, bool positive, float alpha, 
   return positive ? alpha * table_value(999 - idx) : -alpha * table_value(idx);
 }
 
-template <int NP, int NTB, int MT>
+// HW selects the Hogwild instantiation (no delta buffers, no fences, rows prefetched one
+// centre ahead); the general one serves delta and deterministic runs. The row pitch P is
+// d rounded up to 64, so lane slots e < NP-1 always hold valid columns (j+1 < d) and only
+// the last slot is masked.
+template <int NP, int NTB, int MT, bool HW>
 __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout ly) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
   const int g4 = lane >> 4, r16 = lane & 15;
   const int W = a.W, K = a.K, NT = K + 1, P = a.P, d = a.d;
   const int R = 2 * W + 1;
-  const bool delta = a.rd0 != nullptr;
+  const bool delta = HW ? false : a.rd0 != nullptr;
+  const bool fence = HW ? false : a.fence != 0;
+  auto in_p = [&](int e, int j) { return e < NP - 1 || j < P; };
+  auto in_d = [&](int e, int j) { return e < NP - 1 || j < d; };
   unsigned char* wb = lds_raw + (size_t)wib * ly.wave_bytes;
   float* Uf = (float*)wb;
   unsigned char* U0 = wb + ly.off_u0;
@@ -148,7 +155,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
 #pragma unroll
       for (int e = 0; e < NP; e++) {
         const int j = 2 * lane + 128 * e;
-        if (j < P) {
+        if (in_p(e, j)) {
           const uint32_t pr = src[j >> 1];
           float x0 = bf_lo(pr), x1 = bf_hi(pr);
           if (delta) {
@@ -167,7 +174,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
 #pragma unroll
       for (int e = 0; e < NP; e++) {
         const int j = 2 * lane + 128 * e;
-        if (j < d) {
+        if (in_d(e, j)) {
           const float2 u = *(const float2*)(uf + j);
           float x0, x1;
           if (delta) {
@@ -175,12 +182,12 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
             x0 = v.x;
             x1 = v.y;
             atomicAdd(a.out0 + (size_t)tok * P + j, u.x - x0);
-            if (j + 1 < d) atomicAdd(a.out0 + (size_t)tok * P + j + 1, u.y - x1);
+            if (in_d(e, j + 1)) atomicAdd(a.out0 + (size_t)tok * P + j + 1, u.y - x1);
           } else {
             const uint32_t pr = ((const uint32_t*)U0)[(slot * P + j) >> 1];
             x0 = bf_lo(pr);
             x1 = bf_hi(pr);
-            sr_add_pair(a.bsyn0 + (size_t)tok * P + j, u.x - x0, u.y - x1, pr, j + 1 < d, rng);
+            sr_add_pair(a.bsyn0 + (size_t)tok * P + j, u.x - x0, u.y - x1, pr, in_d(e, j + 1), rng);
           }
         }
       }
@@ -211,7 +218,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
     }
     // Hogwild mode loads the next centre's target rows and the next entering context
     // row one centre ahead (packed bf16 pairs); deterministic mode loads in order.
-    const bool pipe = !a.fence && !delta;
+    constexpr bool pipe = HW;
     auto entering_token = [&](int pos) -> int {
       if (pos - chunk0 >= 64) {
         chunk0 += 64;
@@ -232,7 +239,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
 #pragma unroll
       for (int e = 0; e < NP; e++) {
         const int j = 2 * lane + 128 * e;
-        dst[e] = (j < P) ? ((const uint32_t*)(m + (size_t)row * P))[j >> 1] : 0u;
+        dst[e] = in_p(e, j) ? ((const uint32_t*)(m + (size_t)row * P))[j >> 1] : 0u;
       }
     };
     if (pipe) {
@@ -279,7 +286,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
         for (int e = 0; e < NP; e++) {
           const int j = 2 * lane + 128 * e;
           float x0 = 0.0f, x1 = 0.0f;
-          if (k < NT && j < P) {
+          if (k < NT && in_p(e, j)) {
             const uint32_t pr = cT[k][e];
             x0 = bf_lo(pr);
             x1 = bf_hi(pr);
@@ -363,7 +370,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
 #pragma unroll
         for (int e = 0; e < NP; e++) {
           const int j = 2 * lane + 128 * e;
-          if (j < P) {
+          if (in_p(e, j)) {
             const float2 u = *(const float2*)(uf + j);
             float du0 = 0.0f, du1 = 0.0f;
 #pragma unroll
@@ -385,19 +392,19 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
 #pragma unroll
           for (int e = 0; e < NP; e++) {
             const int j = 2 * lane + 128 * e;
-            if (j < d) {
+            if (in_d(e, j)) {
               if (delta) {
                 atomicAdd(a.out1 + (size_t)tk[k] * P + j, dT[k][e][0]);
-                if (j + 1 < d) atomicAdd(a.out1 + (size_t)tk[k] * P + j + 1, dT[k][e][1]);
+                if (in_d(e, j + 1)) atomicAdd(a.out1 + (size_t)tk[k] * P + j + 1, dT[k][e][1]);
               } else {
                 sr_add_pair(a.bsyn1 + (size_t)tk[k] * P + j, dT[k][e][0], dT[k][e][1], cT[k][e],
-                            j + 1 < d, rng);
+                            in_d(e, j + 1), rng);
               }
             }
           }
         }
       }
-      if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+      if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
 
       // ---- ring advance: flush the leaving context, load the entering one
       const int leave = w - W, enter = w + W + 1;
@@ -405,7 +412,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
       if (slot >= R) slot -= R;
       if (leave >= 0) {
         flush_row(slot, readlane_i(ring_tok, slot));
-        if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
       }
       if (enter < L) {
         ring_tok = (lane == slot) ? enter_tok : ring_tok;
@@ -414,7 +421,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
 #pragma unroll
           for (int e = 0; e < NP; e++) {
             const int j = 2 * lane + 128 * e;
-            if (j < P) {
+            if (in_p(e, j)) {
               ((uint32_t*)U0)[(slot * P + j) >> 1] = nE[e];
               *(float2*)(uf + j) = make_float2(bf_lo(nE[e]), bf_hi(nE[e]));
             }
@@ -430,7 +437,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
       const int slot = p % R;
       flush_row(slot, readlane_i(ring_tok, slot));
     }
-    if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     __builtin_amdgcn_wave_barrier();
   }
   if (lane == 0 && a.counters) {
@@ -440,9 +447,9 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
 }
 
 // Instantiations live in mfma_np<NP>.hip (one translation unit per NP).
-const void* mfma_instance_np1(int ntb, int mt);
-const void* mfma_instance_np2(int ntb, int mt);
-const void* mfma_instance_np3(int ntb, int mt);
-const void* mfma_instance_np4(int ntb, int mt);
+const void* mfma_instance_np1(int ntb, int mt, bool hw);
+const void* mfma_instance_np2(int ntb, int mt, bool hw);
+const void* mfma_instance_np3(int ntb, int mt, bool hw);
+const void* mfma_instance_np4(int ntb, int mt, bool hw);
 
 }  // namespace w2v

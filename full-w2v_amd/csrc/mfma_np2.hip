@@ -3,10 +3,14 @@
 
 namespace w2v {
 
-const void* mfma_instance_np2(int ntb, int mt) {
-#define W2V_NT(NTB)                                                          \
-  if (ntb == NTB) return mt == 1 ? (const void*)sgns_mfma_kernel<2, NTB, 1> \
-                                 : (const void*)sgns_mfma_kernel<2, NTB, 2>;
+const void* mfma_instance_np2(int ntb, int mt, bool hw) {
+#define W2V_NT(NTB)                                                                     \
+  if (ntb == NTB) {                                                                     \
+    if (mt == 1) return hw ? (const void*)sgns_mfma_kernel<2, NTB, 1, true>            \
+                           : (const void*)sgns_mfma_kernel<2, NTB, 1, false>;          \
+    return hw ? (const void*)sgns_mfma_kernel<2, NTB, 2, true>                        \
+              : (const void*)sgns_mfma_kernel<2, NTB, 2, false>;                      \
+  }
   W2V_NT(2) W2V_NT(4) W2V_NT(6) W2V_NT(8) W2V_NT(12) W2V_NT(16)
 #undef W2V_NT
   return nullptr;
