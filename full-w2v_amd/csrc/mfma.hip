@@ -3,15 +3,16 @@
 
 namespace w2v {
 
-MfmaLayout mfma_layout(int W, int K, int P, bool delta) {
+MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw) {
   MfmaLayout l;
   const int R = 2 * W + 1, NT = K + 1;
+  const bool u0_regs = u0_in_regs(hw, R > 16 ? 2 : 1, (P + 127) / 128);
   l.PU = P + 4;
   l.PT = P + 8;
   auto al = [](int x) { return (x + 15) / 16 * 16; };
   int o = al(R * l.PU * 4);
   l.off_u0 = o;
-  o = al(o + R * P * (delta ? 4 : 2));
+  if (!u0_regs) o = al(o + R * P * (delta ? 4 : 2));
   l.off_ts = o;
   o = al(o + NT * l.PT * 2);
   l.off_gs = o;
@@ -45,7 +46,7 @@ bool mfma_supported(int d, int W, int K) { return d <= 512 && 2 * W + 1 <= 32 &&
 hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st) {
   const void* fn = mfma_select(a.P, a.K + 1, 2 * a.W + 1, a.rd0 == nullptr && !a.fence);
   if (!fn) return hipErrorInvalidValue;
-  const MfmaLayout ly = mfma_layout(a.W, a.K, a.P, a.rd0 != nullptr);
+  const MfmaLayout ly = mfma_layout(a.W, a.K, a.P, a.rd0 != nullptr, a.rd0 == nullptr && !a.fence);
   const size_t lds = (size_t)ly.wave_bytes * waves_per_block;
   hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
@@ -65,7 +66,7 @@ hipError_t occupancy_mfma(int P, int K, int W, bool delta, int waves_per_block, 
 }
 
 size_t mfma_lds_per_wave(int W, int K, int P, bool delta) {
-  return (size_t)mfma_layout(W, K, P, delta).wave_bytes;
+  return (size_t)mfma_layout(W, K, P, delta, !delta).wave_bytes;
 }
 
 }  // namespace w2v

@@ -33,7 +33,14 @@ struct MfmaLayout {
   int off_u0, off_ts, off_gs, wave_bytes;
 };
 
-MfmaLayout mfma_layout(int W, int K, int P, bool delta);
+// Hogwild runs with one 16-row tile and rows of at most 384 elements keep the load-time
+// copy of the context ring (bf16 pairs) in registers instead of LDS, which lets more
+// waves fit in a CU's LDS.
+__host__ __device__ constexpr bool u0_in_regs(bool hw, int mt, int np) {
+  return hw && mt == 1 && np <= 3;
+}
+
+MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw);
 
 __device__ __forceinline__ float bf_lo(uint32_t p) { return __uint_as_float(p << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
@@ -107,6 +114,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
   const bool fence = HW ? false : a.fence != 0;
   auto in_p = [&](int e, int j) { return e < NP - 1 || j < P; };
   auto in_d = [&](int e, int j) { return e < NP - 1 || j < d; };
+  constexpr bool U0R = u0_in_regs(HW, MT, NP);
   unsigned char* wb = lds_raw + (size_t)wib * ly.wave_bytes;
   float* Uf = (float*)wb;
   unsigned char* U0 = wb + ly.off_u0;
@@ -114,6 +122,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
   float* Gs = (float*)(wb + ly.off_gs);
   const int cpb = K > 0 ? 64 / K : 0;
   unsigned long long words = 0, sents = 0;
+  uint32_t u0r[U0R ? 16 : 1][NP];  // load-time ring pairs when held in registers
 
   for (;;) {
     unsigned s32 = 0;
@@ -149,28 +158,65 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
     int ring_tok = 0;
     int negbuf = 0, nw0 = -(1 << 30);
 
+    // load-time bf16 pairs of ring slot `slot` (wave-uniform): registers or LDS. The
+    // register copy is addressed by selects over all slots, which keeps it out of
+    // private (scratch) memory.
+    auto u0_put = [&](int slot, const uint32_t* v) {
+      if constexpr (U0R) {
+#pragma unroll
+        for (int S = 0; S < 16; S++)
+#pragma unroll
+          for (int e = 0; e < NP; e++) u0r[S][e] = (slot == S) ? v[e] : u0r[S][e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < NP; e++) {
+          const int j = 2 * lane + 128 * e;
+          if (in_p(e, j)) ((uint32_t*)U0)[(slot * P + j) >> 1] = v[e];
+        }
+      }
+    };
+    auto u0_get = [&](int slot, uint32_t* v) {
+      if constexpr (U0R) {
+#pragma unroll
+        for (int e = 0; e < NP; e++) v[e] = u0r[0][e];
+#pragma unroll
+        for (int S = 1; S < 16; S++)
+#pragma unroll
+          for (int e = 0; e < NP; e++) v[e] = (slot == S) ? u0r[S][e] : v[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < NP; e++) {
+          const int j = 2 * lane + 128 * e;
+          v[e] = in_p(e, j) ? ((const uint32_t*)U0)[(slot * P + j) >> 1] : 0u;
+        }
+      }
+    };
     auto load_row = [&](int slot, int tok) {
       const uint32_t* src = (const uint32_t*)(a.bsyn0 + (size_t)tok * P);
       float* uf = Uf + slot * ly.PU;
+      uint32_t prs[NP];
 #pragma unroll
       for (int e = 0; e < NP; e++) {
         const int j = 2 * lane + 128 * e;
+        prs[e] = 0u;
         if (in_p(e, j)) {
           const uint32_t pr = src[j >> 1];
+          prs[e] = pr;
           float x0 = bf_lo(pr), x1 = bf_hi(pr);
           if (delta) {
             x0 += a.rd0[(size_t)tok * P + j];
             x1 += a.rd0[(size_t)tok * P + j + 1];
             *(float2*)((float*)U0 + slot * P + j) = make_float2(x0, x1);
-          } else {
-            ((uint32_t*)U0)[(slot * P + j) >> 1] = pr;
           }
           *(float2*)(uf + j) = make_float2(x0, x1);
         }
       }
+      if (!delta) u0_put(slot, prs);
     };
     auto flush_row = [&](int slot, int tok) {
       const float* uf = Uf + slot * ly.PU;
+      uint32_t prs[NP];
+      if (!delta) u0_get(slot, prs);
 #pragma unroll
       for (int e = 0; e < NP; e++) {
         const int j = 2 * lane + 128 * e;
@@ -184,7 +230,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
             atomicAdd(a.out0 + (size_t)tok * P + j, u.x - x0);
             if (in_d(e, j + 1)) atomicAdd(a.out0 + (size_t)tok * P + j + 1, u.y - x1);
           } else {
-            const uint32_t pr = ((const uint32_t*)U0)[(slot * P + j) >> 1];
+            const uint32_t pr = prs[e];
             x0 = bf_lo(pr);
             x1 = bf_hi(pr);
             sr_add_pair(a.bsyn0 + (size_t)tok * P + j, u.x - x0, u.y - x1, pr, in_d(e, j + 1), rng);
@@ -418,13 +464,11 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
         ring_tok = (lane == slot) ? enter_tok : ring_tok;
         if (pipe) {
           float* uf = Uf + slot * ly.PU;
+          u0_put(slot, nE);
 #pragma unroll
           for (int e = 0; e < NP; e++) {
             const int j = 2 * lane + 128 * e;
-            if (in_p(e, j)) {
-              ((uint32_t*)U0)[(slot * P + j) >> 1] = nE[e];
-              *(float2*)(uf + j) = make_float2(bf_lo(nE[e]), bf_hi(nE[e]));
-            }
+            if (in_p(e, j)) *(float2*)(uf + j) = make_float2(bf_lo(nE[e]), bf_hi(nE[e]));
           }
         } else {
           load_row(slot, enter_tok);
