@@ -122,7 +122,8 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
   float* Gs = (float*)(wb + ly.off_gs);
   const int cpb = K > 0 ? 64 / K : 0;
   unsigned long long words = 0, sents = 0;
-  uint32_t u0r[U0R ? 16 : 1][NP];  // load-time ring pairs when held in registers
+  typedef uint32_t u32x16_t __attribute__((ext_vector_type(16)));
+  u32x16_t u0v[NP];  // load-time ring pairs when held in registers (element = ring slot)
 
   for (;;) {
     unsigned s32 = 0;
@@ -159,14 +160,12 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
     int negbuf = 0, nw0 = -(1 << 30);
 
     // load-time bf16 pairs of ring slot `slot` (wave-uniform): registers or LDS. The
-    // register copy is addressed by selects over all slots, which keeps it out of
-    // private (scratch) memory.
+    // register copy is a 16-element vector per pair indexed by the uniform slot, which
+    // compiles to VGPR-indexed moves (no private memory, no select chains).
     auto u0_put = [&](int slot, const uint32_t* v) {
       if constexpr (U0R) {
 #pragma unroll
-        for (int S = 0; S < 16; S++)
-#pragma unroll
-          for (int e = 0; e < NP; e++) u0r[S][e] = (slot == S) ? v[e] : u0r[S][e];
+        for (int e = 0; e < NP; e++) u0v[e][slot] = v[e];
       } else {
 #pragma unroll
         for (int e = 0; e < NP; e++) {
@@ -178,11 +177,7 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
     auto u0_get = [&](int slot, uint32_t* v) {
       if constexpr (U0R) {
 #pragma unroll
-        for (int e = 0; e < NP; e++) v[e] = u0r[0][e];
-#pragma unroll
-        for (int S = 1; S < 16; S++)
-#pragma unroll
-          for (int e = 0; e < NP; e++) v[e] = (slot == S) ? u0r[S][e] : v[e];
+        for (int e = 0; e < NP; e++) v[e] = u0v[e][slot];
       } else {
 #pragma unroll
         for (int e = 0; e < NP; e++) {
