@@ -20,7 +20,7 @@ hipError_t occupancy_mfma(int P, int K, int W, bool delta, int waves_per_block, 
                           int* blocks_per_cu);
 size_t mfma_lds_per_wave(int W, int K, int P, bool delta);
 bool mfma_supported(int d, int W, int K);
-hipError_t occupancy_plain(int P, int waves_per_block, size_t lds, int* blocks_per_cu);
+hipError_t occupancy_plain(int P, bool delta, int waves_per_block, size_t lds, int* blocks_per_cu);
 size_t plain_lds_per_wave(int W, int P);
 hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
 size_t compact_temp_bytes(int64_t n);
@@ -346,7 +346,7 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
       if (lw * wpb > lmax) continue;
       int bpc = 0;
       hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, p.window, dm != 0, wpb, lw * wpb, &bpc)
-                         : occupancy_plain(c->P, wpb, lw * wpb, &bpc);
+                         : occupancy_plain(c->P, dm != 0, wpb, lw * wpb, &bpc);
       if (oe != hipSuccess || bpc < 1) bpc = 1;
       if (bpc * wpb > best_waves) {
         best_waves = bpc * wpb;

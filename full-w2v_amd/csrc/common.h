@@ -36,6 +36,13 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Marks a register as used here (no instruction is emitted). Placed after prefetched
+// rows have had time to arrive and before a burst of atomics, it makes the compiler's
+// wait for those loads happen at this point, where only loads are outstanding, instead
+// of at the next use, where the in-order memory counter would also cover the atomics.
+__device__ __forceinline__ void settle(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void settle(uint32_t v) { asm volatile("" ::"v"(v)); }
+
 __device__ __forceinline__ int readlane_i(int v, int lane) {
   return __builtin_amdgcn_readlane(v, lane);
 }
@@ -58,6 +65,17 @@ __device__ __forceinline__ int sigmoid_index(float f) {
   x = (x >= 0.0f) ? x : 0.0f;  // saturate, NaN -> 0
   x = (x > 1.0f) ? 1.0f : x;
   return (int)(x * 999.0f);
+}
+
+// Bucket value of the reference's tables evaluated in registers with the generator's
+// formula (generate_expTable_h.cpp:48-51: sigma((2i/1000-1)*6), ends 0 and 1) instead of
+// a memory lookup; nexpTable[j] equals expTable[999-j], so one formula serves both.
+// Values agree with the stored table to a few ulp (device exp and reciprocal).
+__device__ __forceinline__ float table_value(int i) {
+  if (i <= 0) return 0.0f;
+  if (i >= 999) return 1.0f;
+  const float v = __expf((1 - i / 1000.0f * 2) * 6);  // e^-x
+  return __builtin_amdgcn_rcpf(1.0f + v);           // e^x / (e^x + 1)
 }
 
 // splitmix64 finaliser and the keyed stream used for device sampling.

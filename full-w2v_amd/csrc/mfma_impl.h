@@ -81,18 +81,6 @@ __device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, uin
   __builtin_amdgcn_global_atomic_fadd_v2bf16((bf16x2_t*)p, v);
 }
 
-// Table sigmoid with the reference's buckets (index rule of FULL-W2V-kernels.cu:92-99),
-// the bucket value evaluated in registers with the generator's formula
-// (generate_expTable_h.cpp:48-51: sigma((2i/1000-1)*6), ends 0 and 1) instead of a
-// per-lane table gather; nexpTable[j] equals expTable[999-j], so one formula serves both.
-// Values agree with the stored table to a few ulp (device exp and reciprocal).
-__device__ __forceinline__ float table_value(int i) {
-  if (i <= 0) return 0.0f;
-  if (i >= 999) return 1.0f;
-  const float v = __expf((1 - i / 1000.0f * 2) * 6);  // e^-x
-  return __builtin_amdgcn_rcpf(1.0f + v);           // e^x / (e^x + 1)
-}
-
 __device__ __forceinline__ float tile_grad(float f, bool positive, float alpha, const UpdateArgs& a) {
   if (a.sig_exact) return positive ? alpha / (1.0f + expf(f)) : -alpha / (1.0f + expf(-f));
   const int idx = sigmoid_index(f);
@@ -154,10 +142,29 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
     sents += 1;
     uint32_t rng = (uint32_t)key2(a.mixed_seed, 0x5352ull ^ ((uint64_t)a.epoch << 40), (uint64_t)(s * 64 + lane)) | 1u;
 
+    // Token ids and negatives are staged 64 lanes at a time, with the following batch
+    // loaded one batch ahead: the registers read in the centre loop are only ever
+    // written by moves, so no wait on outstanding memory operations is needed there.
     int chunk0 = 0;
     int tokbuf = (lane < L) ? a.tokens[base + lane] : 0;
+    int toknext = (64 + lane < L) ? a.tokens[base + 64 + lane] : 0;
     int ring_tok = 0;
-    int negbuf = 0, nw0 = -(1 << 30);
+    auto neg_batch = [&](int nw) -> int {  // negatives of centres nw .. nw+cpb-1
+      const int cw = nw + lane / K, kk = lane % K;
+      int v = 0;
+      if (lane < cpb * K && cw < L) {
+        if (a.neg) {
+          v = a.neg[(base + cw) * K + kk];
+        } else {
+          const uint64_t x = key2(a.mixed_seed, pos_key(a.epoch, raw + cw), (uint64_t)kk + 1);
+          v = a.unigram[((x >> 32) * (uint64_t)a.n_unigram) >> 32];
+        }
+      }
+      return v;
+    };
+    int nw0 = 0;
+    int negbuf = K > 0 ? neg_batch(0) : 0;
+    int negnext = K > 0 ? neg_batch(cpb) : 0;
 
     // load-time bf16 pairs of ring slot `slot` (wave-uniform): registers or LDS. The
     // register copy is a 16-element vector per pair indexed by the uniform slot, which
@@ -233,23 +240,16 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
         }
       }
     };
-    auto negative = [&](int w, int k) -> int {
-      if (w >= nw0 + cpb) {
-        nw0 = w;
-        const int cw = nw0 + lane / K, kk = lane % K;
-        int v = 0;
-        if (lane < cpb * K && cw < L) {
-          if (a.neg) {
-            v = a.neg[(base + cw) * K + kk];
-          } else {
-            const uint64_t x = key2(a.mixed_seed, pos_key(a.epoch, raw + cw), (uint64_t)kk + 1);
-            v = a.unigram[((x >> 32) * (uint64_t)a.n_unigram) >> 32];
-          }
-        }
-        negbuf = v;
+    // next_centre(w) runs once per centre, in order, before that centre's negatives
+    // are read, so w never passes the end of the staged batches
+    auto next_centre = [&](int w) {
+      if (K > 0 && w >= nw0 + cpb) {
+        nw0 += cpb;
+        negbuf = negnext;
+        negnext = neg_batch(nw0 + cpb);
       }
-      return readlane_i(negbuf, (w - nw0) * K + k);
     };
+    auto negative = [&](int w, int k) -> int { return readlane_i(negbuf, (w - nw0) * K + k); };
 
     const int npre = L < W + 1 ? L : W + 1;
     for (int p = 0; p < npre; p++) {
@@ -260,10 +260,11 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
     // Hogwild mode loads the next centre's target rows and the next entering context
     // row one centre ahead (packed bf16 pairs); deterministic mode loads in order.
     constexpr bool pipe = HW;
-    auto entering_token = [&](int pos) -> int {
+    auto entering_token = [&](int pos) -> int {  // pos advances by one per centre
       if (pos - chunk0 >= 64) {
         chunk0 += 64;
-        tokbuf = (chunk0 + lane < L) ? a.tokens[base + chunk0 + lane] : 0;
+        tokbuf = toknext;
+        toknext = (chunk0 + 64 + lane < L) ? a.tokens[base + chunk0 + 64 + lane] : 0;
       }
       return readlane_i(tokbuf, pos - chunk0);
     };
@@ -310,12 +311,14 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
           for (int e = 0; e < NP; e++) cT[k][e] = nT[k][e];
         }
         if (w + 1 < L) {  // next centre's targets
+          next_centre(w + 1);
           targets_of(w + 1, cs + 1 == R ? 0 : cs + 1, ntk);
 #pragma unroll
           for (int k = 0; k < NTB; k++)
             if (k < NT) fetch_pairs(a.bsyn1, ntk[k], nT[k]);
         }
       } else {
+        next_centre(w);
         targets_of(w, cs, tk);
 #pragma unroll
         for (int k = 0; k < NTB; k++)
@@ -426,6 +429,46 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
         }
       }
 
+      if (pipe) {  // (unconditional: without a prefetch the registers are already settled)
+#pragma unroll
+        for (int k = 0; k < NTB; k++)
+#pragma unroll
+          for (int e = 0; e < NP; e++) settle(nT[k][e]);
+      }
+      const int leave = w - W, enter = w + W + 1;
+      int slot = cs + W + 1;
+      if (slot >= R) slot -= R;
+      // Hogwild: the leaving context's deltas move to registers and the prefetched
+      // entering row takes its slot before any atomic of this centre is issued, so the
+      // wait for that row does not cover them. (Deterministic runs keep flush-then-load:
+      // an entering row must see the leaving row's delta when the word repeats.)
+      float2 lv[NP];
+      uint32_t lp[NP];
+      int ltok = 0;
+      if (pipe) {
+        if (leave >= 0) {
+          ltok = readlane_i(ring_tok, slot);
+          const float* uf = Uf + slot * ly.PU;
+          u0_get(slot, lp);
+#pragma unroll
+          for (int e = 0; e < NP; e++) {
+            const int j = 2 * lane + 128 * e;
+            const float2 u = in_p(e, j) ? *(const float2*)(uf + j) : make_float2(0.0f, 0.0f);
+            lv[e] = make_float2(u.x - bf_lo(lp[e]), u.y - bf_hi(lp[e]));
+          }
+        }
+        if (enter < L) {
+          ring_tok = (lane == slot) ? enter_tok : ring_tok;
+          float* uf = Uf + slot * ly.PU;
+          u0_put(slot, nE);
+#pragma unroll
+          for (int e = 0; e < NP; e++) {
+            const int j = 2 * lane + 128 * e;
+            if (in_p(e, j)) *(float2*)(uf + j) = make_float2(bf_lo(nE[e]), bf_hi(nE[e]));
+          }
+        }
+      }
+
       // ---- target deltas back to the model
 #pragma unroll
       for (int k = 0; k < NTB; k++) {
@@ -448,24 +491,22 @@ __global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout
       if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
 
       // ---- ring advance: flush the leaving context, load the entering one
-      const int leave = w - W, enter = w + W + 1;
-      int slot = cs + W + 1;
-      if (slot >= R) slot -= R;
-      if (leave >= 0) {
-        flush_row(slot, readlane_i(ring_tok, slot));
-        if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-      }
-      if (enter < L) {
-        ring_tok = (lane == slot) ? enter_tok : ring_tok;
-        if (pipe) {
-          float* uf = Uf + slot * ly.PU;
-          u0_put(slot, nE);
+      if (pipe) {
+        if (leave >= 0) {
 #pragma unroll
           for (int e = 0; e < NP; e++) {
             const int j = 2 * lane + 128 * e;
-            if (in_p(e, j)) *(float2*)(uf + j) = make_float2(bf_lo(nE[e]), bf_hi(nE[e]));
+            if (in_d(e, j))
+              sr_add_pair(a.bsyn0 + (size_t)ltok * P + j, lv[e].x, lv[e].y, lp[e], in_d(e, j + 1), rng);
           }
-        } else {
+        }
+      } else {
+        if (leave >= 0) {
+          flush_row(slot, readlane_i(ring_tok, slot));
+          if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        }
+        if (enter < L) {
+          ring_tok = (lane == slot) ? enter_tok : ring_tok;
           load_row(slot, enter_tok);
         }
       }

@@ -20,25 +20,40 @@
 
 namespace w2v {
 
+// REG evaluates the table bucket in registers; otherwise the uploaded tables are read
+// (a vector load whose wait also covers every memory operation in flight, so the
+// Hogwild instantiation uses REG).
+template <bool REG>
 __device__ __forceinline__ float sgns_grad(float f, bool positive, float alpha, const UpdateArgs& a) {
   if (a.sig_exact) {
     return positive ? alpha / (1.0f + expf(f)) : -alpha / (1.0f + expf(-f));
   }
   const int idx = uniform_i(sigmoid_index(f));
+  if (REG) return positive ? alpha * table_value(999 - idx) : -alpha * table_value(idx);
   return positive ? alpha * a.nexp_t[idx] : -alpha * a.exp_t[idx];
 }
 
-__device__ __forceinline__ void det_fence(const UpdateArgs& a) {
-  if (a.fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-}
+// Targets trained together by one wave in the Hogwild instantiation (registers bound
+// the group: T, its delta, the prefetched next rows and the per-pair operands).
+constexpr int plain_group(int E) { return E <= 5 ? 6 : E <= 8 ? 4 : 2; }
 
-template <int E>
+// HW selects the Hogwild instantiation (no delta buffers, no fences, rows prefetched
+// one group ahead). Its targets are trained in groups of G on a skewed schedule: at step
+// s target k of the group trains against context s-k, so the group's wave reductions
+// are independent and overlap. Target k meets context c after target k-1 has updated
+// it and each target meets the contexts left to right, so every element sees the same
+// operations in the same order as the one-pair-at-a-time loop (kernels.cu:151-215).
+template <int E, bool HW>
 __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
+  constexpr int G = HW ? plain_group(E) : 1;
   extern __shared__ float lds[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int W = a.W, K = a.K, P = a.P, d = a.d;
-  const int R = 2 * W + 1;
+  const int R = 2 * W + 1, NT = K + 1;
+  const float* rd0 = HW ? nullptr : a.rd0;
+  const float* rd1 = HW ? nullptr : a.rd1;
+  const bool fence = HW ? false : a.fence != 0;
   float* U0s = lds + (size_t)wib * 2 * R * P;
   float* Ds = U0s + (size_t)R * P;
   const int cpb = K > 0 ? 64 / K : 0;  // centres whose negatives fit in one VGPR
@@ -74,11 +89,29 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
     words += (unsigned long long)L;
     sents += 1;
 
-    // entering-token stream, 64 tokens per VGPR chunk
+    // Token ids and negatives are staged 64 lanes at a time, with the following batch
+    // loaded one batch ahead: the registers read in the centre loop are only ever
+    // written by moves, so no wait on outstanding memory operations is needed there.
     int chunk0 = 0;
     int tokbuf = (lane < L) ? a.tokens[base + lane] : 0;
+    int toknext = (64 + lane < L) ? a.tokens[base + 64 + lane] : 0;
     int ring_tok = 0;  // lane r holds the token id of ring slot r
-    int negbuf = 0, nw0 = -(1 << 30);
+    auto neg_batch = [&](int nw) -> int {  // negatives of centres nw .. nw+cpb-1
+      const int cw = nw + lane / K, kk = lane % K;
+      int v = 0;
+      if (lane < cpb * K && cw < L) {
+        if (a.neg) {
+          v = a.neg[(base + cw) * K + kk];
+        } else {
+          const uint64_t x = key2(a.mixed_seed, pos_key(a.epoch, raw + cw), (uint64_t)kk + 1);
+          v = a.unigram[((x >> 32) * (uint64_t)a.n_unigram) >> 32];
+        }
+      }
+      return v;
+    };
+    int nw0 = 0;
+    int negbuf = K > 0 ? neg_batch(0) : 0;
+    int negnext = K > 0 ? neg_batch(cpb) : 0;
 
     auto load_row = [&](int slot, int tok) {
       const float* src = a.syn0 + (size_t)tok * P;
@@ -87,7 +120,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
 #pragma unroll
       for (int e = 0; e < E; e++) {
         float v = src[e * 64 + lane];
-        if (a.rd0) v += a.rd0[(size_t)tok * P + e * 64 + lane];
+        if (rd0) v += rd0[(size_t)tok * P + e * 64 + lane];
         u0[e * 64 + lane] = v;
         dd[e * 64 + lane] = 0.0f;
       }
@@ -98,31 +131,27 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int j = e * 64 + lane;
-        if (j < d) atomicAdd(dst + j, dd[j]);
+        if (e < E - 1 || j < d) atomicAdd(dst + j, dd[j]);
       }
     };
-    auto negative = [&](int w, int k) -> int {
-      if (w >= nw0 + cpb) {
-        nw0 = w;
-        const int cw = nw0 + lane / K, kk = lane % K;
-        int v = 0;
-        if (lane < cpb * K && cw < L) {
-          if (a.neg) {
-            v = a.neg[(base + cw) * K + kk];
-          } else {
-            const uint64_t x = key2(a.mixed_seed, pos_key(a.epoch, raw + cw), (uint64_t)kk + 1);
-            const uint64_t idx = ((x >> 32) * (uint64_t)a.n_unigram) >> 32;
-            v = a.unigram[idx];
-          }
-        }
-        negbuf = v;
+    // next_centre(w) runs once per centre, in order, before that centre's negatives
+    // are read, so w never passes the end of the staged batches
+    auto next_centre = [&](int w) {
+      if (K > 0 && w >= nw0 + cpb) {
+        nw0 += cpb;
+        negbuf = negnext;
+        negnext = neg_batch(nw0 + cpb);
       }
-      return readlane_i(negbuf, (w - nw0) * K + k);
+    };
+    auto negative = [&](int w, int k) -> int { return readlane_i(negbuf, (w - nw0) * K + k); };
+    // target kg of centre w (0: the centre word, then its negatives)
+    auto target_of = [&](int w, int w_slot, int kg) -> int {
+      return kg == 0 ? readlane_i(ring_tok, w_slot) : negative(w, kg - 1);
     };
 
-    // Hogwild mode loads the next target row and the next entering context row one
-    // step ahead; deterministic mode loads in order after its fences.
-    const bool pipe = !a.fence;
+    // Hogwild mode loads the next group's target rows and the next entering context
+    // row one step ahead; deterministic mode loads in order after its fences.
+    constexpr bool pipe = HW;
     auto fetch = [&](const float* m, const float* dm, int row, float* dst) {
 #pragma unroll
       for (int e = 0; e < E; e++) {
@@ -131,10 +160,11 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
         dst[e] = v;
       }
     };
-    auto entering_token = [&](int pos) -> int {
+    auto entering_token = [&](int pos) -> int {  // pos advances by one per centre
       if (pos - chunk0 >= 64) {
         chunk0 += 64;
-        tokbuf = (chunk0 + lane < L) ? a.tokens[base + chunk0 + lane] : 0;
+        tokbuf = toknext;
+        toknext = (chunk0 + 64 + lane < L) ? a.tokens[base + chunk0 + 64 + lane] : 0;
       }
       return readlane_i(tokbuf, pos - chunk0);
     };
@@ -146,78 +176,160 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       ring_tok = (lane == p) ? tok : ring_tok;
       load_row(p, tok);
     }
-    float nT[E], nE[E];  // prefetched next target row / next entering row
-    int t_next = readlane_i(ring_tok, 0);
-    if (pipe) fetch(a.syn1, a.rd1, t_next, nT);
+    float nT[G][E], nE[E];  // prefetched next group's target rows / next entering row
+    int ntk[G];
+    if (pipe) {
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        ntk[k] = 0;
+        if (k < NT) {
+          ntk[k] = target_of(0, 0, k);
+          fetch(a.syn1, rd1, ntk[k], nT[k]);
+        }
+      }
+    }
 
     int cs = 0;  // ring slot of the centre
     for (int w = 0; w < L; w++) {
       const int lo = w - W < 0 ? 0 : w - W;
       const int hi = w + W > L - 1 ? L - 1 : w + W;
-      const int lo_slot0 = cs - (w - lo) < 0 ? cs - (w - lo) + R : cs - (w - lo);
+      const int nctx = hi - lo;  // window positions other than the centre
       const int enter_pos = w + W + 1;
       int enter_tok = 0;
       if (enter_pos < L) {
         enter_tok = entering_token(enter_pos);
-        if (pipe) fetch(a.syn0, a.rd0, enter_tok, nE);
+        if (pipe) fetch(a.syn0, rd0, enter_tok, nE);
       }
-      for (int k = 0; k <= K; k++) {
-        const int t = t_next;
-        float T0[E], DT[E];
+      // ring slot of the i-th context (positions lo..hi without the centre)
+      auto ctx_slot = [&](int i) {
+        const int p = lo + i + (lo + i >= w ? 1 : 0);
+        int sl = cs + (p - w);
+        if (sl < 0) sl += R;
+        if (sl >= R) sl -= R;
+        return sl;
+      };
+      int g0 = 0;
+      do {  // (at least one target per centre)
+        const int nk = NT - g0 < G ? NT - g0 : G;
+        float T0[G][E], DT[G][E];
+        int tk[G];
         if (pipe) {
 #pragma unroll
-          for (int e = 0; e < E; e++) T0[e] = nT[e];
+          for (int k = 0; k < G; k++) {
+            tk[k] = ntk[k];
+#pragma unroll
+            for (int e = 0; e < E; e++) T0[k][e] = nT[k][e];
+          }
+          // next group: the rest of this centre's targets, else the next centre's
+          int nw = w, nslot = cs, ng0 = g0 + G;
+          if (ng0 >= NT) {
+            nw = w + 1;
+            nslot = cs + 1 == R ? 0 : cs + 1;
+            ng0 = 0;
+          }
+          if (nw < L) {
+            next_centre(nw);
+#pragma unroll
+            for (int k = 0; k < G; k++) {
+              if (ng0 + k < NT) {
+                ntk[k] = target_of(nw, nslot, ng0 + k);
+                fetch(a.syn1, rd1, ntk[k], nT[k]);
+              }
+            }
+          }
         } else {
-          fetch(a.syn1, a.rd1, t, T0);
-        }
+          next_centre(w);
 #pragma unroll
-        for (int e = 0; e < E; e++) DT[e] = 0.0f;
-        // next target: the next negative of this centre, or the next centre's word
-        if (k < K) t_next = negative(w, k);
-        else if (w + 1 < L) t_next = readlane_i(ring_tok, cs + 1 == R ? 0 : cs + 1);
-        if (pipe && (k < K || w + 1 < L)) fetch(a.syn1, a.rd1, t_next, nT);
-        int slot = lo_slot0;
-        for (int c = lo; c <= hi; c++, slot = (slot + 1 == R) ? 0 : slot + 1) {
-          if (c == w) continue;
-          float* u0 = U0s + (size_t)slot * P;
-          float* dd = Ds + (size_t)slot * P;
-          float uu[E], tt[E], dv[E];
-          float part = 0.0f;
-#pragma unroll
-          for (int e = 0; e < E; e++) {
-            dv[e] = dd[e * 64 + lane];
-            uu[e] = u0[e * 64 + lane] + dv[e];
-            tt[e] = T0[e] + DT[e];
-            part = fmaf(uu[e], tt[e], part);
-          }
-          const float f = wave_sum(part);
-          const float g = sgns_grad(f, k == 0, alpha, a);
-#pragma unroll
-          for (int e = 0; e < E; e++) {
-            dd[e * 64 + lane] = fmaf(g, tt[e], dv[e]);  // context and target move together
-            DT[e] = fmaf(g, uu[e], DT[e]);
+          for (int k = 0; k < G; k++) {
+            tk[k] = 0;
+            if (k < nk) {
+              tk[k] = target_of(w, cs, g0 + k);
+              fetch(a.syn1, rd1, tk[k], T0[k]);
+            }
           }
         }
-        float* tdst = a.out1 + (size_t)t * P;
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-          const int j = e * 64 + lane;
-          if (j < d) atomicAdd(tdst + j, DT[e]);
+        for (int k = 0; k < G; k++)
+#pragma unroll
+          for (int e = 0; e < E; e++) DT[k][e] = 0.0f;
+
+        for (int st = 0; st < nctx + nk - 1; st++) {
+          bool act[G];
+          int sl[G];
+          float part[G], uu[G][E], dv[G][E];
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            const int i = st - k;
+            act[k] = k < nk && i >= 0 && i < nctx;
+            sl[k] = 0;
+            part[k] = 0.0f;
+            if (act[k]) {
+              sl[k] = ctx_slot(i);
+              const float* u0 = U0s + (size_t)sl[k] * P;
+              const float* dd = Ds + (size_t)sl[k] * P;
+#pragma unroll
+              for (int e = 0; e < E; e++) {
+                dv[k][e] = dd[e * 64 + lane];
+                uu[k][e] = u0[e * 64 + lane] + dv[k][e];
+                part[k] = fmaf(uu[k][e], T0[k][e] + DT[k][e], part[k]);
+              }
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < G; k++) part[k] = wave_sum(part[k]);
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            if (act[k]) {
+              const float g = sgns_grad<HW>(part[k], g0 + k == 0, alpha, a);
+              float* dd = Ds + (size_t)sl[k] * P;
+#pragma unroll
+              for (int e = 0; e < E; e++) {
+                const float tt = T0[k][e] + DT[k][e];
+                dd[e * 64 + lane] = fmaf(g, tt, dv[k][e]);  // context and target move together
+                DT[k][e] = fmaf(g, uu[k][e], DT[k][e]);
+              }
+            }
+          }
         }
-        det_fence(a);
-      }
+        if (pipe) {  // prefetched rows settle before this group's atomics are issued
+#pragma unroll
+          for (int k = 0; k < G; k++)
+#pragma unroll
+            for (int e = 0; e < E; e++) settle(nT[k][e]);
+#pragma unroll
+          for (int e = 0; e < E; e++) settle(nE[e]);
+        }
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+          if (k < nk) {
+            float* tdst = a.out1 + (size_t)tk[k] * P;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+              const int j = e * 64 + lane;
+              if (e < E - 1 || j < d) atomicAdd(tdst + j, DT[k][e]);
+            }
+          }
+        }
+        if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        g0 += G;
+      } while (g0 < NT);
       // ring advance: flush the leaving context, then load the entering one into the
       // same slot (kernels.cu:217-238)
       const int leave = w - W, enter = w + W + 1;
       int slot = cs + W + 1;
       if (slot >= R) slot -= R;
-      if (leave >= 0) {
-        flush_row(slot, readlane_i(ring_tok, slot));
-        det_fence(a);
-      }
-      if (enter < L) {
-        ring_tok = (lane == slot) ? enter_tok : ring_tok;
-        if (pipe) {
+      if (pipe) {
+        // Hogwild: the leaving deltas move to registers and the prefetched entering row
+        // takes the slot before the flush atomics are issued
+        float lv[E];
+        int ltok = 0;
+        if (leave >= 0) {
+          ltok = readlane_i(ring_tok, slot);
+#pragma unroll
+          for (int e = 0; e < E; e++) lv[e] = Ds[(size_t)slot * P + e * 64 + lane];
+        }
+        if (enter < L) {
+          ring_tok = (lane == slot) ? enter_tok : ring_tok;
           float* u0 = U0s + (size_t)slot * P;
           float* dd = Ds + (size_t)slot * P;
 #pragma unroll
@@ -225,7 +337,22 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
             u0[e * 64 + lane] = nE[e];
             dd[e * 64 + lane] = 0.0f;
           }
-        } else {
+        }
+        if (leave >= 0) {
+          float* dst = a.out0 + (size_t)ltok * P;
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const int j = e * 64 + lane;
+            if (e < E - 1 || j < d) atomicAdd(dst + j, lv[e]);
+          }
+        }
+      } else {
+        if (leave >= 0) {
+          flush_row(slot, readlane_i(ring_tok, slot));
+          if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        }
+        if (enter < L) {
+          ring_tok = (lane == slot) ? enter_tok : ring_tok;
           load_row(slot, enter_tok);
         }
       }
@@ -236,7 +363,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       const int slot = p % R;
       flush_row(slot, readlane_i(ring_tok, slot));
     }
-    det_fence(a);
+    if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
   }
   if (lane == 0 && a.counters) {
     atomicAdd(a.counters + 0, words);
@@ -248,23 +375,28 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
 size_t plain_lds_per_wave(int W, int P) { return (size_t)2 * (2 * W + 1) * P * sizeof(float); }
 
 template <int E>
-static hipError_t launch_plain_e(const UpdateArgs& a, int waves_per_block, int blocks,
-                                 hipStream_t st) {
-  const size_t lds = plain_lds_per_wave(a.W, a.P) * waves_per_block;
-  hipError_t err = hipFuncSetAttribute((const void*)sgns_plain_kernel<E>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(sgns_plain_kernel<E>, dim3(blocks), dim3(64 * waves_per_block), lds, st, a);
-  return hipGetLastError();
+static const void* plain_fn(bool hw) {
+  return hw ? (const void*)sgns_plain_kernel<E, true> : (const void*)sgns_plain_kernel<E, false>;
 }
 
 template <int E>
-static hipError_t occupancy_plain_e(int waves_per_block, size_t lds, int* blocks_per_cu) {
-  hipError_t err = hipFuncSetAttribute((const void*)sgns_plain_kernel<E>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+static hipError_t launch_plain_e(const UpdateArgs& a, int waves_per_block, int blocks,
+                                 hipStream_t st) {
+  const void* fn = plain_fn<E>(a.rd0 == nullptr && a.rd1 == nullptr && !a.fence);
+  const size_t lds = plain_lds_per_wave(a.W, a.P) * waves_per_block;
+  hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sgns_plain_kernel<E>,
-                                                      64 * waves_per_block, lds);
+  UpdateArgs aa = a;
+  void* args[] = {&aa};
+  return hipLaunchKernel(fn, dim3(blocks), dim3(64 * waves_per_block), args, lds, st);
+}
+
+template <int E>
+static hipError_t occupancy_plain_e(bool hw, int waves_per_block, size_t lds, int* blocks_per_cu) {
+  const void* fn = plain_fn<E>(hw);
+  hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (err != hipSuccess) return err;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 64 * waves_per_block, lds);
 }
 
 #define W2V_E_CASES(M) M(1) M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12) M(13) M(14) M(15) M(16)
@@ -279,10 +411,10 @@ hipError_t launch_plain(const UpdateArgs& a, int waves_per_block, int blocks, hi
   }
 }
 
-hipError_t occupancy_plain(int P, int waves_per_block, size_t lds, int* blocks_per_cu) {
+hipError_t occupancy_plain(int P, bool delta, int waves_per_block, size_t lds, int* blocks_per_cu) {
   switch (P / 64) {
 #define CASE(E) \
-  case E: return occupancy_plain_e<E>(waves_per_block, lds, blocks_per_cu);
+  case E: return occupancy_plain_e<E>(!delta, waves_per_block, lds, blocks_per_cu);
     W2V_E_CASES(CASE)
 #undef CASE
     default: return hipErrorInvalidValue;
