@@ -482,6 +482,113 @@ int w2v_model_download(w2v_ctx* c, float* syn0, float* syn1neg) {
   return W2V_OK;
 }
 
+static const char kCkptMagic[8] = {'F', 'W', '2', 'V', 'C', 'K', 'P', 'T'};
+static const uint32_t kCkptVersion = 1, kCkptHeader = 64;
+
+// header: magic, version, header bytes, then the fields at fixed offsets (64 bytes)
+static void ckpt_pack(const w2v_checkpoint& m, unsigned char* h) {
+  memset(h, 0, kCkptHeader);
+  memcpy(h, kCkptMagic, 8);
+  memcpy(h + 8, &kCkptVersion, 4);
+  memcpy(h + 12, &kCkptHeader, 4);
+  memcpy(h + 16, &m.vocab_size, 8);
+  memcpy(h + 24, &m.layer_size, 4);
+  memcpy(h + 28, &m.epochs_done, 4);
+  memcpy(h + 32, &m.words_done, 8);
+  memcpy(h + 40, &m.seed, 8);
+  memcpy(h + 48, &m.alpha, 4);
+  memcpy(h + 52, &m.n_rng, 4);
+}
+
+static bool ckpt_unpack(const unsigned char* h, w2v_checkpoint* m) {
+  uint32_t ver, hb;
+  memcpy(&ver, h + 8, 4);
+  memcpy(&hb, h + 12, 4);
+  if (memcmp(h, kCkptMagic, 8) != 0 || ver != kCkptVersion || hb != kCkptHeader) return false;
+  memcpy(&m->vocab_size, h + 16, 8);
+  memcpy(&m->layer_size, h + 24, 4);
+  memcpy(&m->epochs_done, h + 28, 4);
+  memcpy(&m->words_done, h + 32, 8);
+  memcpy(&m->seed, h + 40, 8);
+  memcpy(&m->alpha, h + 48, 4);
+  memcpy(&m->n_rng, h + 52, 4);
+  return m->vocab_size > 0 && m->layer_size > 0 && m->n_rng >= 0;
+}
+
+int w2v_checkpoint_read_header(const char* path, w2v_checkpoint* meta) {
+  if (!path || !meta) return fail(nullptr, W2V_ERR_ARG, "path and meta are required");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(nullptr, W2V_ERR_ARG, "cannot open %s", path);
+  unsigned char h[64];
+  const bool ok = fread(h, 1, sizeof(h), f) == sizeof(h) && ckpt_unpack(h, meta);
+  fclose(f);
+  return ok ? W2V_OK : fail(nullptr, W2V_ERR_ARG, "%s is not a checkpoint of this format", path);
+}
+
+int w2v_checkpoint_save(w2v_ctx* c, const char* path, const w2v_checkpoint* meta,
+                        const uint64_t* rng) {
+  if (!c || !path || !meta || (meta->n_rng > 0 && !rng) || meta->n_rng < 0)
+    return fail(c, W2V_ERR_ARG, "checkpoint arguments");
+  if (c->delta_mode) return fail(c, W2V_ERR_STATE, "checkpoints are not taken in delta mode");
+  const size_t V = c->p.vocab_size, d = c->p.layer_size;
+  std::vector<float> m((size_t)V * d);
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return fail(c, W2V_ERR_ARG, "cannot create %s", tmp.c_str());
+  w2v_checkpoint h = *meta;
+  h.vocab_size = (int64_t)V;
+  h.layer_size = (int32_t)d;
+  h.seed = c->p.seed;
+  unsigned char hb[64];
+  ckpt_pack(h, hb);
+  bool ok = fwrite(hb, 1, sizeof(hb), f) == sizeof(hb);
+  if (ok && h.n_rng > 0) ok = fwrite(rng, 8, (size_t)h.n_rng, f) == (size_t)h.n_rng;
+  int rc = W2V_OK;
+  for (int which = 0; ok && which < 2; which++) {
+    if ((rc = w2v_model_download(c, which == 0 ? m.data() : nullptr, which == 1 ? m.data() : nullptr)))
+      break;
+    ok = fwrite(m.data(), 4, m.size(), f) == m.size();
+  }
+  ok = (fclose(f) == 0) && ok;
+  if (rc || !ok || rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    return rc ? rc : fail(c, W2V_ERR_ARG, "writing %s failed", path);
+  }
+  return W2V_OK;
+}
+
+int w2v_checkpoint_load(w2v_ctx* c, const char* path, w2v_checkpoint* meta, uint64_t* rng,
+                        int32_t rng_capacity) {
+  if (!c || !path || !meta || rng_capacity < 0 || (rng_capacity > 0 && !rng))
+    return fail(c, W2V_ERR_ARG, "checkpoint arguments");
+  if (c->delta_mode) return fail(c, W2V_ERR_STATE, "checkpoints are not loaded in delta mode");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(c, W2V_ERR_ARG, "cannot open %s", path);
+  unsigned char hb[64];
+  w2v_checkpoint h;
+  if (fread(hb, 1, sizeof(hb), f) != sizeof(hb) || !ckpt_unpack(hb, &h)) {
+    fclose(f);
+    return fail(c, W2V_ERR_ARG, "%s is not a checkpoint of this format", path);
+  }
+  if (h.vocab_size != c->p.vocab_size || h.layer_size != c->p.layer_size) {
+    fclose(f);
+    return fail(c, W2V_ERR_ARG, "checkpoint is V=%lld d=%d, the context V=%lld d=%d",
+                (long long)h.vocab_size, h.layer_size, (long long)c->p.vocab_size, c->p.layer_size);
+  }
+  std::vector<uint64_t> states((size_t)h.n_rng);
+  bool ok = h.n_rng == 0 || fread(states.data(), 8, states.size(), f) == states.size();
+  const size_t n = (size_t)h.vocab_size * h.layer_size;
+  std::vector<float> s0(n), s1(n);
+  ok = ok && fread(s0.data(), 4, n, f) == n && fread(s1.data(), 4, n, f) == n;
+  fclose(f);
+  if (!ok) return fail(c, W2V_ERR_ARG, "%s is truncated", path);
+  int rc = w2v_model_upload(c, s0.data(), s1.data());
+  if (rc) return rc;
+  for (int i = 0; i < h.n_rng && i < rng_capacity; i++) rng[i] = states[i];
+  *meta = h;
+  return W2V_OK;
+}
+
 int w2v_set_delta_mode(w2v_ctx* c, int32_t on) {
   if (!c) return W2V_ERR_ARG;
   int rc = w2v_sync_all(c);

@@ -40,7 +40,7 @@ struct Options {
       kernel_batch = 200, layer = 100, msl = 1000;
   float alpha = 0.025f, sample = 1e-3f;
   // options of this implementation
-  std::string precision = "fp32";
+  std::string precision = "fp32", save_ckpt, resume;
   int device_sampling = 0, device = 0, exact_sigmoid = 0;
   unsigned long long seed = 1;
 };
@@ -83,6 +83,8 @@ void usage(const char* prog) {
   printf("\t-device-sampling <0|1>\n\t\tSubsample and draw negatives on the GPU from a resident corpus\n");
   printf("\t-exact-sigmoid <0|1>\n\t\tUse 1/(1+exp(-x)) instead of the 1000-entry table\n");
   printf("\t-device <int>\n\t-seed <int>\n");
+  printf("\t-save-checkpoint <file>\n\t\tWrite model, syn1neg and counters to <file> at every epoch end\n");
+  printf("\t-resume <file>\n\t\tContinue from a checkpoint (same vocabulary, sizes and -threads)\n");
   printf("\nExamples:\n./%s -train data.txt -output vec.txt -size 128 -window 5 -sample 1e-4 -negative 5 -epoch 3\n\n",
          prog);
 }
@@ -142,6 +144,8 @@ Options parse(int argc, char** argv) {
       break;
     }
   str("-precision", o.precision);
+  str("-save-checkpoint", o.save_ckpt);
+  str("-resume", o.resume);
   num("-device-sampling", o.device_sampling);
   num("-exact-sigmoid", o.exact_sigmoid);
   num("-device", o.device);
@@ -192,6 +196,10 @@ struct Shared {
   uint64_t wca = 0;
   float alpha = 0.025f;
   double t_start = 0, t_end = 0;
+  int epochs_done = 0;             // completed before this run (resume)
+  bool resumed = false;
+  uint64_t wca0 = 0;               // words counted before this run
+  std::vector<uint64_t> rng;       // per-thread LCG states: resume input, checkpoint output
   Corpus* shuffle_target;
   std::atomic<int> failed{0};
 };
@@ -227,7 +235,7 @@ void* train_thread(void* p) {
   const std::vector<int64_t>& start = sh.corpus->start;
   const int32_t* uni = sh.unigram->data();
   const float* subs = sh.subs->empty() ? nullptr : sh.subs->data();
-  uint64_t next_random = (uint64_t)id, word_count = 0, last_word_count = 0;
+  uint64_t next_random = sh.resumed ? sh.rng[id] : (uint64_t)id, word_count = 0, last_word_count = 0;
   int64_t ss = beg, cur = 0;
   std::vector<int32_t> btok, bneg;
   std::vector<int64_t> boff(1, 0);
@@ -258,7 +266,7 @@ void* train_thread(void* p) {
   };
   pthread_barrier_wait(&sh.bar);
   if (id == 0) sh.t_start = now_s();
-  for (int local_epoch = o.epochs; local_epoch > 0 && !sh.failed;) {
+  for (int local_epoch = o.epochs - sh.epochs_done; local_epoch > 0 && !sh.failed;) {
     if (word_count - last_word_count > 10000) {  // driver.cu:168-190
       const uint64_t wca = merge_words();
       if (o.debug > 0 && id == 0) {
@@ -307,6 +315,27 @@ void* train_thread(void* p) {
       ss = beg;
       cur = 0;
       local_epoch--;
+      if (!o.save_ckpt.empty()) {
+        // epoch boundary: this epoch's sentences trained, all threads here, then thread 0
+        // writes the model, syn1neg, the word count, alpha and every thread's LCG state
+        submit();
+        for (int st = 0; st < o.streams; st++) w2v_sync(sh.ctx, id * o.streams + st);
+        sh.rng[id] = next_random;
+        pthread_barrier_wait(&sh.bar);
+        if (id == 0) {
+          w2v_checkpoint m{};
+          m.epochs_done = o.epochs - local_epoch;
+          m.words_done = sh.wca;
+          m.alpha = sh.alpha;
+          m.n_rng = o.threads;
+          const int rc = w2v_checkpoint_save(sh.ctx, o.save_ckpt.c_str(), &m, sh.rng.data());
+          if (rc) {
+            fprintf(stderr, "w2v_checkpoint_save failed (%d): %s\n", rc, w2v_last_error(sh.ctx));
+            sh.failed = 1;
+          }
+        }
+        pthread_barrier_wait(&sh.bar);
+      }
       if (o.epoch_shuffle) {
         pthread_barrier_wait(&sh.bar);
         if (id == 0 && local_epoch > 0) sh.shuffle_target->shuffle();
@@ -419,6 +448,24 @@ int main(int argc, char** argv) {
   }
   if (o.debug > 0) fprintf(stderr, "Initializing network\n");
   if ((rc = w2v_init_model(ctx))) die("w2v_init_model", ctx, rc);
+  w2v_checkpoint resumed{};
+  std::vector<uint64_t> resumed_rng(o.threads);
+  if (!o.resume.empty()) {
+    if ((rc = w2v_checkpoint_load(ctx, o.resume.c_str(), &resumed, resumed_rng.data(), o.threads)))
+      die("w2v_checkpoint_load", ctx, rc);
+    if (!o.device_sampling && resumed.n_rng != o.threads) {
+      fprintf(stderr, "checkpoint holds %d thread states, -threads is %d\n", resumed.n_rng, o.threads);
+      return 1;
+    }
+    if (resumed.epochs_done >= o.epochs) {
+      fprintf(stderr, "checkpoint has %d epochs done, -epoch is %d: nothing to train\n",
+              resumed.epochs_done, o.epochs);
+      return 1;
+    }
+    if (o.debug > 0)
+      fprintf(stderr, "Resuming after epoch %d (%llu words, alpha %f)\n", resumed.epochs_done,
+              (unsigned long long)resumed.words_done, resumed.alpha);
+  }
   std::vector<uint64_t> cn(V);
   for (int i = 0; i < V; i++) cn[i] = vocab.words[i].cn;
   std::vector<int32_t> unigram(kUnigramTableSize);
@@ -456,7 +503,7 @@ int main(int argc, char** argv) {
     if ((rc = w2v_sync_all(ctx))) die("w2v_sync_all", ctx, rc);
     t0 = now_s();
     int turn = 0;
-    for (int e = 0; e < o.epochs; e++) {
+    for (int e = resumed.epochs_done; e < o.epochs; e++) {
       for (int64_t sb = 0; sb < n_sent;) {
         int64_t se = sb + 1;
         while (se < n_sent && off[se + 1] - off[sb] <= chunk_tokens) se++;
@@ -476,6 +523,20 @@ int main(int argc, char** argv) {
       }
       if (o.epoch_shuffle && e + 1 < o.epochs && o.debug > 0)
         fprintf(stderr, "\n-epoch-shuffle is not applied on the device-sampling path\n");
+      if (!o.save_ckpt.empty()) {
+        if ((rc = w2v_sync_all(ctx))) die("w2v_sync_all", ctx, rc);
+        w2v_stats st;
+        w2v_get_stats(ctx, &st);
+        w2v_checkpoint m{};
+        m.epochs_done = e + 1;
+        m.words_done = resumed.words_done + st.words_read;
+        // the device schedule at the boundary (driver.cu:188-189)
+        const double wca = (double)(e + 1) * vocab.train_words;
+        float a = o.alpha * (float)(1 - wca / ((double)o.epochs * vocab.train_words + 1));
+        m.alpha = a < o.alpha * 0.0001f ? o.alpha * 0.0001f : a;
+        if ((rc = w2v_checkpoint_save(ctx, o.save_ckpt.c_str(), &m, nullptr)))
+          die("w2v_checkpoint_save", ctx, rc);
+      }
     }
     if ((rc = w2v_sync_all(ctx))) die("w2v_sync_all", ctx, rc);
     t1 = now_s();
@@ -492,6 +553,14 @@ int main(int argc, char** argv) {
     sh.train_words = vocab.train_words;
     sh.n_sent = n_sent;
     sh.alpha = o.alpha;
+    sh.rng.assign(o.threads, 0);
+    if (!o.resume.empty()) {
+      sh.resumed = true;
+      sh.epochs_done = resumed.epochs_done;
+      sh.wca = sh.wca0 = resumed.words_done;
+      sh.alpha = resumed.alpha;
+      sh.rng = resumed_rng;
+    }
     sh.shuffle_target = &corpus;
     pthread_mutex_init(&sh.mu, nullptr);
     pthread_barrier_init(&sh.bar, nullptr, o.threads);
@@ -503,7 +572,7 @@ int main(int argc, char** argv) {
     }
     for (int i = 0; i < o.threads; i++) pthread_join(th[i], nullptr);
     if (sh.failed) return 1;
-    words = sh.wca;
+    words = sh.wca - sh.wca0;
     t0 = sh.t_start;
     t1 = sh.t_end;
   }

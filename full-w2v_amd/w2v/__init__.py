@@ -24,7 +24,7 @@ MODE = {"hogwild": 0, "deterministic": 1}
 # Every function declared in include/w2v.h, in declaration order.
 EXPORTS = [
     "w2v_create", "w2v_destroy", "w2v_last_error", "w2v_model_upload", "w2v_model_download",
-    "w2v_sigmoid_tables", "w2v_submit_padded", "w2v_submit_csr", "w2v_sync", "w2v_sync_all",
+    "w2v_checkpoint_save", "w2v_checkpoint_load", "w2v_checkpoint_read_header", "w2v_sigmoid_tables", "w2v_submit_padded", "w2v_submit_csr", "w2v_sync", "w2v_sync_all",
     "w2v_set_delta_mode", "w2v_delta_download", "w2v_tables_upload", "w2v_set_schedule",
     "w2v_corpus_upload", "w2v_train_device", "w2v_train_chunk", "w2v_get_stats", "w2v_heldout_loss",
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
@@ -52,6 +52,12 @@ class Synth(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class Checkpoint(ctypes.Structure):
+    _fields_ = [("vocab_size", ctypes.c_int64), ("layer_size", ctypes.c_int32),
+                ("epochs_done", ctypes.c_int32), ("words_done", ctypes.c_uint64),
+                ("seed", ctypes.c_uint64), ("alpha", ctypes.c_float), ("n_rng", ctypes.c_int32)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("words_read", ctypes.c_uint64), ("words_trained", ctypes.c_uint64),
                 ("sentences", ctypes.c_uint64), ("launches", ctypes.c_uint64)]
@@ -73,6 +79,9 @@ def lib() -> ctypes.CDLL:
             "w2v_destroy": [vp],
             "w2v_model_upload": [vp, vp, vp],
             "w2v_model_download": [vp, vp, vp],
+            "w2v_checkpoint_save": [vp, ctypes.c_char_p, ctypes.POINTER(Checkpoint), vp],
+            "w2v_checkpoint_load": [vp, ctypes.c_char_p, ctypes.POINTER(Checkpoint), vp, i32],
+            "w2v_checkpoint_read_header": [ctypes.c_char_p, ctypes.POINTER(Checkpoint)],
             "w2v_sigmoid_tables": [vp, vp],
             "w2v_submit_padded": [vp, i32, vp, vp, vp, vp, i32],
             "w2v_submit_csr": [vp, i32, vp, vp, i64, vp, vp],
@@ -118,6 +127,15 @@ class W2VError(RuntimeError):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def checkpoint_header(path):
+    """Header fields of a checkpoint file (no device needed)."""
+    m = Checkpoint()
+    rc = lib().w2v_checkpoint_read_header(os.fsencode(path), ctypes.byref(m))
+    if rc:
+        raise W2VError(rc, lib().w2v_last_error(None).decode())
+    return {k: getattr(m, k) for k, _ in Checkpoint._fields_}
 
 
 def unigram_table(counts, n=100_000_000, corrected=False):
@@ -292,6 +310,22 @@ class Trainer:
         w, b = ctypes.c_int32(), ctypes.c_int32()
         self._chk(lib().w2v_get_launch_shape(self.h, ctypes.byref(w), ctypes.byref(b)))
         return w.value, b.value
+
+    def checkpoint_save(self, path, epochs_done, words_done, alpha, rng=()):
+        """Model + counters at an epoch boundary (see include/w2v.h for the layout)."""
+        r = np.ascontiguousarray(rng, np.uint64)
+        m = Checkpoint(0, 0, int(epochs_done), int(words_done), 0, float(alpha), len(r))
+        self._chk(lib().w2v_checkpoint_save(self.h, os.fsencode(path), ctypes.byref(m),
+                                            _ptr(r) if len(r) else None))
+
+    def checkpoint_load(self, path, rng_capacity=64):
+        """Loads the model; returns (meta dict, rng states)."""
+        m = Checkpoint()
+        r = np.zeros(rng_capacity, np.uint64)
+        self._chk(lib().w2v_checkpoint_load(self.h, os.fsencode(path), ctypes.byref(m),
+                                            _ptr(r) if rng_capacity else None, int(rng_capacity)))
+        meta = {k: getattr(m, k) for k, _ in Checkpoint._fields_}
+        return meta, r[:min(m.n_rng, rng_capacity)].copy()
 
     def kernel_time(self, stream=0):
         ms = ctypes.c_double()

@@ -87,6 +87,35 @@ int w2v_model_upload(w2v_ctx* ctx, const float* syn0, const float* syn1neg);
  * Synchronises all streams first. Either pointer may be NULL. */
 int w2v_model_download(w2v_ctx* ctx, float* syn0, float* syn1neg);
 
+/* Training state at an epoch boundary (the "syn1neg and counters at epoch boundaries,
+ * so training can resume" extension of saveModel, improved_driver_functions.cpp:225-242,
+ * which exports syn0 only). */
+typedef struct w2v_checkpoint {
+  int64_t vocab_size;
+  int32_t layer_size;
+  int32_t epochs_done;  /* completed epochs */
+  uint64_t words_done;  /* word_count_actual: corpus tokens read, subsampled ones included */
+  uint64_t seed;        /* w2v_params.seed of the run (filled in by w2v_checkpoint_save) */
+  float alpha;          /* learning rate in force at the boundary */
+  int32_t n_rng;        /* host LCG states stored with the checkpoint (one per thread) */
+} w2v_checkpoint;
+
+/* Writes meta (epochs_done, words_done, alpha, n_rng; V, d and seed come from the
+ * context), the n_rng states rng[0..n_rng) and both matrices (FP32, V x d row-major,
+ * unpadded; bf16 storage converts exactly) to path, through a temporary file renamed into
+ * place. File layout, little-endian: "FW2VCKPT", u32 version (1), u32 header bytes (64),
+ * i64 V, i32 d, i32 epochs_done, u64 words_done, u64 seed, f32 alpha, i32 n_rng,
+ * 16 zero bytes, u64 rng[n_rng], f32 syn0[V*d], f32 syn1neg[V*d].
+ * W2V_ERR_STATE while delta mode is on. */
+int w2v_checkpoint_save(w2v_ctx* ctx, const char* path, const w2v_checkpoint* meta,
+                        const uint64_t* rng);
+/* Reads a checkpoint into the model (V and d must match the context) and returns its
+ * meta; up to rng_capacity LCG states are copied to rng (may be NULL when 0). */
+int w2v_checkpoint_load(w2v_ctx* ctx, const char* path, w2v_checkpoint* meta, uint64_t* rng,
+                        int32_t rng_capacity);
+/* Header only; needs no context or device. */
+int w2v_checkpoint_read_header(const char* path, w2v_checkpoint* meta);
+
 /* The sigmoid tables the kernels use: expTable/nexpTable of the reference
  * (expTable.h, generated by generate_expTable_h.cpp:24-74 with size 1000). */
 int w2v_sigmoid_tables(float* exp_table, float* nexp_table);
