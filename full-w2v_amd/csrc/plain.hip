@@ -23,6 +23,13 @@ namespace w2v {
 // REG evaluates the table bucket in registers; otherwise the uploaded tables are read
 // (a vector load whose wait also covers every memory operation in flight, so the
 // Hogwild instantiation uses REG).
+// Per-lane form (register-evaluated table) for gradients of several targets at once.
+__device__ __forceinline__ float lane_grad(float f, bool positive, float alpha, const UpdateArgs& a) {
+  if (a.sig_exact) return positive ? alpha / (1.0f + expf(f)) : -alpha / (1.0f + expf(-f));
+  const int idx = sigmoid_index(f);
+  return positive ? alpha * table_value(999 - idx) : -alpha * table_value(idx);
+}
+
 template <bool REG>
 __device__ __forceinline__ float sgns_grad(float f, bool positive, float alpha, const UpdateArgs& a) {
   if (a.sig_exact) {
@@ -277,10 +284,22 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
           }
 #pragma unroll
           for (int k = 0; k < G; k++) part[k] = wave_sum(part[k]);
+          // the group's gradients in one lane-parallel evaluation: lane k takes target k
+          float gk[G];
+          if constexpr (G > 1) {
+            float fl = part[0];
+#pragma unroll
+            for (int k = 1; k < G; k++) fl = (lane == k) ? part[k] : fl;
+            const float gl = lane_grad(fl, g0 + lane == 0, alpha, a);
+#pragma unroll
+            for (int k = 0; k < G; k++) gk[k] = __int_as_float(readlane_i(__float_as_int(gl), k));
+          } else {
+            gk[0] = sgns_grad<HW>(part[0], g0 == 0, alpha, a);
+          }
 #pragma unroll
           for (int k = 0; k < G; k++) {
             if (act[k]) {
-              const float g = sgns_grad<HW>(part[k], g0 + k == 0, alpha, a);
+              const float g = gk[k];
               float* dd = Ds + (size_t)sl[k] * P;
 #pragma unroll
               for (int e = 0; e < E; e++) {
