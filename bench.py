@@ -147,6 +147,48 @@ def run_variant(w2v, kernel, storage, steps, warmup, host):
                 nn_recovery=rec, ms_per_step=secs / max(steps, 1) * 1e3)
 
 
+def run_streamed(w2v, steps, warmup, host):
+    """Same training with the corpus in host memory: each step hands one chunk of host
+    tokens to w2v_train_chunk, which copies it through pinned staging and trains it on one
+    of two alternating streams (the copy of one chunk overlaps training of the other).
+    Words/sec here includes the host-to-device transfers."""
+    V, d, W, K = WL["V"], WL["d"], WL["W"], WL["K"]
+    tr = w2v.Trainer(V, d, W, K, msl=WL["len_max"], storage="bf16", kernel="mfma",
+                     sigmoid="table", mode="hogwild", seed=WL["seed"], streams=2)
+    cfg = w2v.synth_config(WL["clusters"], WL["median"], WL["len_max"], WL["zipf"], WL["stay"],
+                           WL["in_cluster"], WL["sigma"], WL["seed"])
+    tr.synth_corpus(cfg, WL["n_sent"])
+    nt, ns = tr.corpus_info()
+    tok, off = tr.corpus_download(0, ns)  # the corpus in host memory
+    tr.tables_upload(host["unigram"], host["subs"])
+    epochs = max(1, math.ceil((steps + warmup) / CHUNKS))
+    tr.set_schedule(WL["alpha"], epochs, nt)
+    tr.init_model()
+    bounds = [round(i * ns / CHUNKS) for i in range(CHUNKS + 1)]
+
+    def step(i):
+        e, c = divmod(i, CHUNKS)
+        b0, b1 = bounds[c], bounds[c + 1]
+        tr.train_chunk(e, tok[off[b0]:off[b1]], off[b0:b1 + 1] - off[b0], int(off[b0]), stream=i % 2)
+
+    for i in range(warmup):
+        step(i)
+    tr.sync()
+    st0 = tr.stats()
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        step(i)
+        if (i - warmup) % 5 == 4:
+            log(f"[streamed] step {i - warmup + 1}/{steps} {time.perf_counter() - t0:.1f}s")
+    tr.sync()
+    secs = time.perf_counter() - t0
+    words = tr.stats()["words_read"] - st0["words_read"]
+    tr.close()
+    log(f"[streamed] {words / secs / 1e6:.1f} M words/s with host-to-device transfers")
+    return dict(words_per_sec=words / secs, seconds=secs, words=int(words),
+                host_bytes_per_step=float(4 * words / max(steps, 1)))
+
+
 def cpu_baseline(host, target_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
@@ -215,6 +257,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="all", help="all | plain | mfma")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--streamed", action="store_true",
+                    help="also time the MFMA path with the corpus streamed from host memory")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to scale HBM byte counters")
     args = ap.parse_args()
@@ -234,6 +278,7 @@ def main():
     results = {}
     for kernel, storage in vs:
         results[kernel] = run_variant(w2v, kernel, storage, args.steps, args.warmup, host)
+    streamed = run_streamed(w2v, args.steps, args.warmup, host) if args.streamed else None
     cpu = None if args.no_cpu else cpu_baseline(host)
     best = max(results.values(), key=lambda r: r["words_per_sec"])
     line = {
@@ -274,6 +319,8 @@ def main():
                 / results["plain"]["heldout_loss"]
     if cpu:
         line["gpu_vs_cpu"] = best["words_per_sec"] / cpu["value"]
+    if streamed:
+        line["streamed_from_host"] = streamed
     print(json.dumps(line), flush=True)
 
 

@@ -280,8 +280,9 @@ def distinct_batch(V, lens, K, rng):
 
 
 def test_hogwild_path_one_wave_matches_oracle(orc, w2vmod):
-    """Hogwild mode (no fences, rows loaded one step ahead) on a single wave: with no row
-    revisited it must equal the serialised FP64 update to the deterministic tolerance."""
+    """Hogwild mode with delta buffers (the general instantiation without fences) on a
+    single wave: with no row revisited it must equal the serialised FP64 update to the
+    deterministic tolerance."""
     V, d, W, K = 4000, 128, 5, 5
     rng = np.random.default_rng(31)
     syn0, syn1 = model(V, d, rng)
@@ -300,3 +301,44 @@ def test_hogwild_path_one_wave_matches_oracle(orc, w2vmod):
     e0, e1 = row_rel_err(g0, r0), row_rel_err(g1, r1)
     print(f"hogwild path, 1 wave: syn0 rel {e0:.2e} syn1 rel {e1:.2e}", flush=True)
     assert e0 <= TOL and e1 <= TOL
+
+
+@pytest.mark.parametrize("sigmoid", ["exact", "table"])
+def test_hogwild_fast_path_one_wave_matches_oracle(orc, w2vmod, sigmoid):
+    """The Hogwild instantiation (no delta buffers: target groups on the skewed schedule,
+    register-evaluated sigmoid buckets, rows prefetched a group ahead) on one wave, rows
+    never revisited, updating the FP32 model in place: the model change must equal the
+    serialised FP64 update. The in-place FP32 add rounds row + delta once per element; that
+    half ulp of the stored value is taken off each element's error before the row-relative
+    check. Table-sigmoid trials may straddle a bucket edge (FP32 vs FP64 dot product) now
+    and then."""
+    def stored_rel_err(after, before, ref):
+        got = after.astype(np.float64) - before.astype(np.float64)
+        half_ulp = np.spacing(np.abs(after)).astype(np.float64) / 2
+        scale = np.abs(ref).max(axis=1)
+        touched = scale > 0
+        assert np.all(got[~touched] == 0), "GPU wrote rows the oracle never touched"
+        err = np.maximum(np.abs(got - ref) - half_ulp, 0).max(axis=1)
+        return float((err[touched] / scale[touched]).max())
+
+    V, d, W, K = 6000, 300, 5, 5
+    trials = 4 if sigmoid == "table" else 1
+    passed = 0
+    for t in range(trials):
+        rng = np.random.default_rng(61 + t)
+        syn0, syn1 = model(V, d, rng)
+        lens = [20, 1, 7, 40, 12, 3]
+        tokens, off, neg = distinct_batch(V, lens, K, rng)
+        alpha = np.full(len(lens), 0.25, np.float32)
+        tr = w2vmod.Trainer(V, d, W, K, msl=64, sigmoid=sigmoid, mode="hogwild")
+        tr.set_launch_shape(1, 1)
+        tr.upload(syn0, syn1)
+        tr.submit_csr(tokens, off, neg, alpha)
+        a0, a1 = tr.download()
+        tr.close()
+        r0, r1 = oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha,
+                              sig_mode=1 if sigmoid == "exact" else 0)
+        e0, e1 = stored_rel_err(a0, syn0, r0), stored_rel_err(a1, syn1, r1)
+        print(f"hogwild fast path ({sigmoid}), 1 wave: syn0 rel {e0:.2e} syn1 rel {e1:.2e}", flush=True)
+        passed += e0 <= TOL and e1 <= TOL
+    assert passed >= trials - (1 if sigmoid == "table" else 0)
