@@ -50,6 +50,7 @@ CASES = [
     (30, 256, 1, 0, [9]),
     (40, 128, 8, 5, [50]),      # 2W+1 = 17: two 16-row MFMA tiles
     (60, 300, 10, 15, [45, 3]),  # the largest survey shape (W=10, K=15, d=300)
+    (20000, 300, 10, 15, [4000]),  # ... with the longest survey sentence (-msl 4000)
 ]
 
 

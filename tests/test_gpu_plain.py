@@ -77,6 +77,7 @@ CASES = [
     (30, 64, 10, 15, [25]),
     (200, 128, 5, 5, [130]),
     (25, 32, 3, 2, [5, 0, 17, 3]),
+    (20000, 300, 10, 15, [4000]),  # the longest survey sentence (-msl 4000) at W=10 K=15 d=300
 ]
 
 
