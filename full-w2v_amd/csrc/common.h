@@ -74,8 +74,9 @@ __device__ __forceinline__ int sigmoid_index(float f) {
 __device__ __forceinline__ float table_value(int i) {
   if (i <= 0) return 0.0f;
   if (i >= 999) return 1.0f;
-  const float v = __expf((1 - i / 1000.0f * 2) * 6);  // e^-x
-  return __builtin_amdgcn_rcpf(1.0f + v);           // e^x / (e^x + 1)
+  // e^-x with x = (2i/1000 - 1)*6, as 2^(6*log2(e) - i*0.012*log2(e)): one FMA and v_exp_f32
+  const float v = __builtin_amdgcn_exp2f(fmaf((float)i, -0.0173123404f, 8.65616989f));
+  return __builtin_amdgcn_rcpf(1.0f + v);  // e^x / (e^x + 1)
 }
 
 // splitmix64 finaliser and the keyed stream used for device sampling.

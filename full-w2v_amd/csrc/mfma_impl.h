@@ -71,8 +71,9 @@ __device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, uin
   const u16x2_t seven = {(unsigned short)(7u << 7), (unsigned short)(7u << 7)};
   ex = ex > seven ? ex - seven : u16x2_t{0, 0};  // 2^(e-7) per half, 0 for tiny values
   const uint32_t ul = __builtin_bit_cast(uint32_t, ex);
-  const float ua = __uint_as_float(0x3F800000u | ((r >> 16) << 7)) - 1.5f;  // 16 bits each
-  const float ub = __uint_as_float(0x3F800000u | ((r & 0xFFFFu) << 7)) - 1.5f;
+  // 16 random bits each as the top mantissa bits of a float in [1, 2)
+  const float ua = __uint_as_float(0x3F800000u | ((r >> 9) & 0x7FFF80u)) - 1.5f;
+  const float ub = __uint_as_float(0x3F800000u | ((r << 7) & 0x7FFF80u)) - 1.5f;
   da = fmaf(ua, __uint_as_float(ul << 16), da);
   db = vb ? fmaf(ub, __uint_as_float(ul & 0xFFFF0000u), db) : 0.0f;
   bf16x2_t v;
@@ -91,8 +92,12 @@ __device__ __forceinline__ float tile_grad(float f, bool positive, float alpha, 
 // centre ahead); the general one serves delta and deterministic runs. The row pitch P is
 // d rounded up to 64, so lane slots e < NP-1 always hold valid columns (j+1 < d) and only
 // the last slot is masked.
+// Register state grows with NP x NTB; up to 18 it fits the 256 registers that let two
+// waves share a SIMD, which the launch bound holds the compiler to (larger shapes would
+// spill under it and run one wave per SIMD instead).
 template <int NP, int NTB, int MT, bool HW>
-__global__ void __launch_bounds__(256) sgns_mfma_kernel(UpdateArgs a, MfmaLayout ly) {
+__global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
+    sgns_mfma_kernel(UpdateArgs a, MfmaLayout ly) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
   const int g4 = lane >> 4, r16 = lane & 15;
