@@ -60,6 +60,9 @@ struct StreamState {
   DevBuf d_tok, d_neg, d_len, d_alpha, d_off;
   // device-corpus step buffers
   DevBuf flags, pos, kept, kept_off, temp;
+  // graph of a captured epoch (w2v_train_epoch_graph) and the step bounds it was built for
+  hipGraphExec_t gexec = nullptr;
+  std::vector<int64_t> gkey;
   // timing
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   double ms = 0.0;
@@ -220,7 +223,7 @@ UpdateArgs base_args(w2v_ctx* c) {
 }
 
 // Launches the update kernel for a prepared argument block on one stream.
-int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a) {
+int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a, bool timed = true) {
   if (a.n_sent <= 0) return W2V_OK;
   if (a.n_sent >= (int64_t)1 << 31)
     return fail(c, W2V_ERR_ARG, "at most 2^31-1 sentences per launch (got %lld)", (long long)a.n_sent);
@@ -236,6 +239,11 @@ int launch_update(w2v_ctx* c, StreamState& s, UpdateArgs& a) {
   if (c->p.mode == W2V_MODE_DETERMINISTIC) {
     wpb = 1;
     blocks = 1;
+  }
+  if (!timed) {  // inside a stream capture: the launch only, timed around the whole graph
+    if (c->p.kernel == W2V_KERNEL_MFMA) HIP_TRY(c, launch_mfma(a, wpb, blocks, s.st));
+    else HIP_TRY(c, launch_plain(a, wpb, blocks, s.st));
+    return W2V_OK;
   }
   hipEvent_t e0, e1;
   HIP_TRY(c, hipEventCreate(&e0));
@@ -329,7 +337,8 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
     return bail(fail(c, W2V_ERR_NOMEM, "table allocation failed"));
   if (hipMemcpy(c->d_exp, e, sizeof(e), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_nexp, ne, sizeof(ne), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(c->d_counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess)
+      hipMemset(c->d_counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess)  // the context's streams do not wait for these
     return bail(fail(c, W2V_ERR_HIP, "table upload failed"));
   // launch shape of the update kernel, for normal and delta mode: waves per block from
   // the per-wave LDS ring, persistent grid from the occupancy at that LDS size
@@ -384,6 +393,7 @@ int w2v_destroy(w2v_ctx* c) {
                       &s.kept_off, &s.temp})
       if (d->p) hipFree(d->p);
     if (s.queue) hipFree(s.queue);
+    if (s.gexec) hipGraphExecDestroy(s.gexec);
     if (s.st) hipStreamDestroy(s.st);
   }
   for (void* q : {(void*)c->syn0, (void*)c->syn1, (void*)c->bsyn0, (void*)c->bsyn1, (void*)c->dsyn0, (void*)c->dsyn1, (void*)c->d_exp,
@@ -469,6 +479,7 @@ int w2v_model_upload(w2v_ctx* c, const float* syn0, const float* syn1neg) {
   const bool bf = c->p.storage == W2V_STORAGE_BF16;
   if (syn0 && (rc = copy_matrix(c, bf ? (void*)c->bsyn0 : (void*)c->syn0, syn0, true))) return rc;
   if (syn1neg && (rc = copy_matrix(c, bf ? (void*)c->bsyn1 : (void*)c->syn1, syn1neg, true))) return rc;
+  HIP_TRY(c, hipDeviceSynchronize());  // copies complete before work on the context's streams
   return W2V_OK;
 }
 
@@ -599,8 +610,11 @@ int w2v_set_delta_mode(w2v_ctx* c, int32_t on) {
       if (hipMalloc(&c->dsyn0, rows) != hipSuccess || hipMalloc(&c->dsyn1, rows) != hipSuccess)
         return fail(c, W2V_ERR_NOMEM, "delta buffer allocation failed");
     }
-    HIP_TRY(c, hipMemset(c->dsyn0, 0, rows));
-    HIP_TRY(c, hipMemset(c->dsyn1, 0, rows));
+    // cleared in the order of stream 0 and waited for: the training streams are
+    // non-blocking and would not wait for the null stream
+    HIP_TRY(c, hipMemsetAsync(c->dsyn0, 0, rows, c->ss[0].st));
+    HIP_TRY(c, hipMemsetAsync(c->dsyn1, 0, rows, c->ss[0].st));
+    HIP_TRY(c, hipStreamSynchronize(c->ss[0].st));
   }
   c->delta_mode = on ? 1 : 0;
   return W2V_OK;
@@ -727,6 +741,7 @@ int w2v_tables_upload(w2v_ctx* c, const int32_t* unigram, int64_t n_unigram, con
     if (hipMalloc(&c->d_subs, b) != hipSuccess) return fail(c, W2V_ERR_NOMEM, "subsample table allocation failed");
     HIP_TRY(c, hipMemcpy(c->d_subs, subsample_p, b, hipMemcpyHostToDevice));
   }
+  HIP_TRY(c, hipDeviceSynchronize());  // copies complete before work on the context's streams
   return W2V_OK;
 }
 
@@ -756,29 +771,30 @@ int w2v_corpus_upload(w2v_ctx* c, const int32_t* tokens, int64_t n_tokens, const
     return fail(c, W2V_ERR_NOMEM, "corpus allocation failed");
   if (n_tokens) HIP_TRY(c, hipMemcpy(c->d_corpus, tokens, (size_t)n_tokens * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_off, off, (size_t)(n_sent + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipDeviceSynchronize());  // copies complete before work on the context's streams
   c->h_off.assign(off, off + n_sent + 1);
   c->n_tokens = n_tokens;
   c->n_sent = n_sent;
   return W2V_OK;
 }
 
-int w2v_train_device(w2v_ctx* c, int32_t stream, int32_t epoch, int64_t sb, int64_t se) {
-  int rc = check_stream(c, stream);
-  if (rc) return rc;
-  if (!c->d_corpus) return fail(c, W2V_ERR_STATE, "no device corpus uploaded");
-  if (c->p.negative > 0 && !c->d_unigram) return fail(c, W2V_ERR_STATE, "no unigram table uploaded");
-  if (sb < 0 || se > c->n_sent || sb > se) return fail(c, W2V_ERR_ARG, "sentence range out of bounds");
-  if (epoch < 0 || epoch >= c->epochs) return fail(c, W2V_ERR_ARG, "epoch outside the schedule");
-  if (sb == se) return W2V_OK;
-  const int64_t tb = c->h_off[sb], n = c->h_off[se] - tb;
-  if (n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "at most 2^31-2 tokens per call");
-  StreamState& s = c->ss[stream];
-  HIP_TRY(c, hipSetDevice(c->p.device));
+// Buffers of one device-path step of up to n corpus tokens in up to ns sentences.
+static int ensure_step_buffers(w2v_ctx* c, StreamState& s, int64_t n, int64_t ns) {
+  int rc;
   const size_t tb_bytes = compact_temp_bytes(n);
   if ((rc = ensure_dev(c, s.flags, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.pos, (size_t)(n + 1) * 4)) ||
-      (rc = ensure_dev(c, s.kept, (size_t)(n + 1) * 4)) ||
-      (rc = ensure_dev(c, s.kept_off, (size_t)(se - sb + 1) * 8)) || (rc = ensure_dev(c, s.temp, tb_bytes)))
+      (rc = ensure_dev(c, s.kept, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.kept_off, (size_t)(ns + 1) * 8)) ||
+      (rc = ensure_dev(c, s.temp, tb_bytes)))
     return rc;
+  return W2V_OK;
+}
+
+// One device-path step on the stream: keep flags, their scan, kept tokens and kept
+// sentence offsets for sentences [sb, se), then the update kernel. Allocates nothing
+// (ensure_step_buffers first), so it can be recorded into a graph.
+static int enqueue_device_step(w2v_ctx* c, StreamState& s, int32_t epoch, int64_t sb, int64_t se,
+                               bool timed) {
+  const int64_t tb = c->h_off[sb], n = c->h_off[se] - tb;
   UpdateArgs a = base_args(c);
   HIP_TRY(c, launch_compact(c->d_corpus, c->d_off, sb, se, c->d_subs, a.mixed_seed, epoch,
                             (int32_t*)s.flags.p, (int32_t*)s.pos.p, (int32_t*)s.kept.p,
@@ -795,8 +811,118 @@ int w2v_train_device(w2v_ctx* c, int32_t stream, int32_t epoch, int64_t sb, int6
   a.total_words = (uint64_t)c->epochs * c->train_words + 1;
   a.epoch_words = (uint64_t)epoch * c->train_words;
   a.n_sent = se - sb;
+  return launch_update(c, s, a, timed);
+}
+
+static int check_device_training(w2v_ctx* c, int32_t stream, int32_t epoch) {
+  int rc = check_stream(c, stream);
+  if (rc) return rc;
+  if (!c->d_corpus) return fail(c, W2V_ERR_STATE, "no device corpus uploaded");
+  if (c->p.negative > 0 && !c->d_unigram) return fail(c, W2V_ERR_STATE, "no unigram table uploaded");
+  if (epoch < 0 || epoch >= c->epochs) return fail(c, W2V_ERR_ARG, "epoch outside the schedule");
+  return W2V_OK;
+}
+
+int w2v_train_device(w2v_ctx* c, int32_t stream, int32_t epoch, int64_t sb, int64_t se) {
+  int rc = check_device_training(c, stream, epoch);
+  if (rc) return rc;
+  if (sb < 0 || se > c->n_sent || sb > se) return fail(c, W2V_ERR_ARG, "sentence range out of bounds");
+  if (sb == se) return W2V_OK;
+  const int64_t n = c->h_off[se] - c->h_off[sb];
+  if (n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "at most 2^31-2 tokens per call");
+  StreamState& s = c->ss[stream];
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  if ((rc = ensure_step_buffers(c, s, n, se - sb))) return rc;
+  if ((rc = enqueue_device_step(c, s, epoch, sb, se, true))) return rc;
   c->words_read += (uint64_t)n;
-  return launch_update(c, s, a);
+  return W2V_OK;
+}
+
+int w2v_train_epoch_graph(w2v_ctx* c, int32_t stream, int32_t epoch, const int64_t* bounds,
+                          int32_t n_steps) {
+  int rc = check_device_training(c, stream, epoch);
+  if (rc) return rc;
+  if (!bounds || n_steps < 1) return fail(c, W2V_ERR_ARG, "need at least one step");
+  int64_t max_tok = 0, max_sent = 0, total = 0, run = 0;
+  for (int32_t i = 0; i < n_steps; i++) {
+    const int64_t sb = bounds[i], se = bounds[i + 1];
+    if (sb < 0 || se > c->n_sent || sb > se) return fail(c, W2V_ERR_ARG, "step %d: sentence range out of bounds", i);
+    const int64_t n = c->h_off[se] - c->h_off[sb];
+    if (n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "step %d: at most 2^31-2 tokens per step", i);
+    max_tok = n > max_tok ? n : max_tok;
+    max_sent = se - sb > max_sent ? se - sb : max_sent;
+    total += n;
+    run += sb < se ? 1 : 0;  // empty steps launch nothing, as in w2v_train_device
+  }
+  if (total == 0) return W2V_OK;
+  StreamState& s = c->ss[stream];
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  if ((rc = ensure_step_buffers(c, s, max_tok, max_sent))) return rc;
+  // record the epoch's steps
+  HIP_TRY(c, hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal));
+  for (int32_t i = 0; i < n_steps && rc == W2V_OK; i++)
+    if (bounds[i] < bounds[i + 1]) rc = enqueue_device_step(c, s, epoch, bounds[i], bounds[i + 1], false);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(s.st, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (ce != hipSuccess || !g)
+    return fail(c, W2V_ERR_HIP, "stream capture of the epoch failed: %s", hipGetErrorString(ce));
+  // memset nodes have given wrong results on replays: the graph must hold kernels only
+  size_t nn = 0;
+  hipError_t ge = hipGraphGetNodes(g, nullptr, &nn);
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (ge == hipSuccess && nn) ge = hipGraphGetNodes(g, nodes.data(), &nn);
+  for (size_t i = 0; ge == hipSuccess && i < nn; i++) {
+    hipGraphNodeType t;
+    ge = hipGraphNodeGetType(nodes[i], &t);
+    if (ge == hipSuccess && t != hipGraphNodeTypeKernel) {
+      hipGraphDestroy(g);
+      return fail(c, W2V_ERR_UNSUPPORTED, "captured epoch holds a non-kernel node (type %d)", (int)t);
+    }
+  }
+  if (ge != hipSuccess) {
+    hipGraphDestroy(g);
+    return fail(c, W2V_ERR_HIP, "graph inspection failed: %s", hipGetErrorString(ge));
+  }
+  // the epoch is a kernel argument: refresh the cached executable for the same steps,
+  // instantiate a new one otherwise
+  std::vector<int64_t> key(bounds, bounds + n_steps + 1);
+  if (s.gexec && s.gkey == key) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(s.gexec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      hipGraphExecDestroy(s.gexec);
+      s.gexec = nullptr;
+    }
+  } else if (s.gexec) {
+    hipGraphExecDestroy(s.gexec);
+    s.gexec = nullptr;
+  }
+  if (!s.gexec) {
+    const hipError_t ie = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      s.gexec = nullptr;
+      hipGraphDestroy(g);
+      return fail(c, W2V_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+  }
+  hipGraphDestroy(g);
+  s.gkey = key;
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  HIP_TRY(c, hipEventRecord(e0, s.st));
+  HIP_TRY(c, hipGraphLaunch(s.gexec, s.st));
+  HIP_TRY(c, hipEventRecord(e1, s.st));
+  s.pending.emplace_back(e0, e1);
+  s.launches += (uint64_t)run;
+  c->launches += (uint64_t)run;
+  c->words_read += (uint64_t)total;
+  return W2V_OK;
 }
 
 int w2v_train_chunk(w2v_ctx* c, int32_t stream, int32_t epoch, const int32_t* tokens,
@@ -912,10 +1038,11 @@ int w2v_heldout_loss(w2v_ctx* c, const int32_t* tokens, const int64_t* off, int6
     return fail(c, W2V_ERR_NOMEM, "held-out buffers allocation failed");
   }
   hipError_t e = hipSuccess;
-  if (nt) e = hipMemcpy(dt, tokens, (size_t)nt * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(doff, off, (size_t)(n_sent + 1) * 8, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(acc, 0, 8);
-  if (e == hipSuccess) e = hipMemset(np, 0, 8);
+  // uploads in the order of the stream the kernel runs on
+  if (nt) e = hipMemcpyAsync(dt, tokens, (size_t)nt * 4, hipMemcpyHostToDevice, c->ss[0].st);
+  if (e == hipSuccess) e = hipMemcpyAsync(doff, off, (size_t)(n_sent + 1) * 8, hipMemcpyHostToDevice, c->ss[0].st);
+  if (e == hipSuccess) e = hipMemsetAsync(acc, 0, 8, c->ss[0].st);  // ordered before the kernel
+  if (e == hipSuccess) e = hipMemsetAsync(np, 0, 8, c->ss[0].st);
   if (e == hipSuccess)
     e = launch_heldout_loss(c->syn0, c->syn1, c->bsyn0, c->bsyn1, c->P, c->p.window, c->p.negative, dt, doff, n_sent,
                             c->d_unigram, c->n_unigram, mix64(c->p.seed), acc, np, c->num_cu, c->ss[0].st);
@@ -966,12 +1093,13 @@ int w2v_init_model(w2v_ctx* c) {
   lcg_jump_tables(ja, jc);
   uint64_t* dj = nullptr;
   if (hipMalloc(&dj, sizeof(ja) + sizeof(jc)) != hipSuccess) return fail(c, W2V_ERR_NOMEM, "jump table");
-  hipError_t e = hipMemcpy(dj, ja, sizeof(ja), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(dj + 64, jc, sizeof(jc), hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpyAsync(dj, ja, sizeof(ja), hipMemcpyHostToDevice, c->ss[0].st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dj + 64, jc, sizeof(jc), hipMemcpyHostToDevice, c->ss[0].st);
   const bool bf = c->p.storage == W2V_STORAGE_BF16;
   const size_t rows = (size_t)c->p.vocab_size * c->P * (bf ? 2 : 4);
-  if (e == hipSuccess) e = hipMemset(bf ? (void*)c->bsyn0 : (void*)c->syn0, 0, rows);
-  if (e == hipSuccess) e = hipMemset(bf ? (void*)c->bsyn1 : (void*)c->syn1, 0, rows);
+  // clears ordered before the initialisation kernel on the same stream
+  if (e == hipSuccess) e = hipMemsetAsync(bf ? (void*)c->bsyn0 : (void*)c->syn0, 0, rows, c->ss[0].st);
+  if (e == hipSuccess) e = hipMemsetAsync(bf ? (void*)c->bsyn1 : (void*)c->syn1, 0, rows, c->ss[0].st);
   if (e == hipSuccess)
     e = launch_init_syn0(c->syn0, c->bsyn0, c->p.vocab_size, c->p.layer_size, c->P, dj, dj + 64,
                          c->ss[0].st);
@@ -1048,12 +1176,13 @@ int w2v_synth_corpus(w2v_ctx* c, const w2v_synth* cfg, int64_t first, int64_t n)
     return rc;
   }
   char* base = (char*)tab.p;
-  hipError_t e = hipMemcpy(base, t.gthr.data(), b_g, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(base + o_m, t.members.data(), b_m, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(base + o_cs, t.cstart.data(), b_cs, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(base + o_ct, t.cthr.data(), b_ct, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(base + o_l, t.lthr.data(), b_l, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(len.p, 0, (size_t)(n + 1) * 8);
+  // uploads in the order of the stream the generator kernels run on
+  hipError_t e = hipMemcpyAsync(base, t.gthr.data(), b_g, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + o_m, t.members.data(), b_m, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + o_cs, t.cstart.data(), b_cs, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + o_ct, t.cthr.data(), b_ct, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + o_l, t.lthr.data(), b_l, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(len.p, 0, (size_t)(n + 1) * 8, st);
   if (e == hipSuccess)
     e = launch_synth_lengths(n, first, ms, (const uint64_t*)(base + o_l), cfg->len_max, (int64_t*)len.p, st);
   if (e == hipSuccess && hipMalloc(&c->d_off, (size_t)(n + 1) * 8) != hipSuccess) e = hipErrorOutOfMemory;
@@ -1118,7 +1247,7 @@ int w2v_corpus_counts(w2v_ctx* c, uint64_t* counts) {
   const size_t b = (size_t)c->p.vocab_size * 8;
   unsigned long long* dc = nullptr;
   if (hipMalloc(&dc, b) != hipSuccess) return fail(c, W2V_ERR_NOMEM, "count buffer");
-  hipError_t e = hipMemset(dc, 0, b);
+  hipError_t e = hipMemsetAsync(dc, 0, b, c->ss[0].st);  // ordered before the count kernel
   if (e == hipSuccess && c->n_tokens > 0) e = launch_count(c->d_corpus, c->n_tokens, dc, c->num_cu, c->ss[0].st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->ss[0].st);
   if (e == hipSuccess) e = hipMemcpy(counts, dc, b, hipMemcpyDeviceToHost);

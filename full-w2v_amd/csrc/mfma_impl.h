@@ -51,11 +51,12 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
   return __builtin_bit_cast(uint32_t, v);
 }
 
-__device__ __forceinline__ uint32_t xorshift(uint32_t& s) {
-  s ^= s << 13;
-  s ^= s >> 17;
-  s ^= s << 5;
-  return s;
+// Dither source: a per-lane Weyl sequence with one xor-shift, 32 bits per pair. The two
+// 16-bit fields used for the elements of a pair (bits 7..22, and 23..31 with 0..6) are
+// disjoint, and each element's dither is uniform over its updates.
+__device__ __forceinline__ uint32_t next_bits(uint32_t& w) {
+  w += 0x9E3779B9u;
+  return w ^ (w >> 15);
 }
 
 // Stochastically rounded add of (da, db) to the bf16 pair at p. xp is the pair as last
@@ -66,16 +67,17 @@ typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void sr_add_pair(uint16_t* p, float da, float db, uint32_t xp, bool vb,
                                             uint32_t& rng) {
-  const uint32_t r = xorshift(rng);
+  const uint32_t r = next_bits(rng);
   u16x2_t ex = __builtin_bit_cast(u16x2_t, xp & 0x7F807F80u);
   const u16x2_t seven = {(unsigned short)(7u << 7), (unsigned short)(7u << 7)};
   ex = ex > seven ? ex - seven : u16x2_t{0, 0};  // 2^(e-7) per half, 0 for tiny values
   const uint32_t ul = __builtin_bit_cast(uint32_t, ex);
   // 16 random bits each as the top mantissa bits of a float in [1, 2)
-  const float ua = __uint_as_float(0x3F800000u | ((r >> 9) & 0x7FFF80u)) - 1.5f;
-  const float ub = __uint_as_float(0x3F800000u | ((r << 7) & 0x7FFF80u)) - 1.5f;
+  const float ua = __uint_as_float((r & 0x7FFF80u) | 0x3F800000u) - 1.5f;
+  const float ub = __uint_as_float((__builtin_amdgcn_alignbit(r, r, 16) & 0x7FFF80u) | 0x3F800000u) - 1.5f;
   da = fmaf(ua, __uint_as_float(ul << 16), da);
-  db = vb ? fmaf(ub, __uint_as_float(ul & 0xFFFF0000u), db) : 0.0f;
+  db = fmaf(ub, __uint_as_float(ul & 0xFFFF0000u), db);
+  db = vb ? db : 0.0f;
   bf16x2_t v;
   v[0] = (__bf16)da;
   v[1] = (__bf16)db;

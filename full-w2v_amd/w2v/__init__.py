@@ -26,7 +26,7 @@ EXPORTS = [
     "w2v_create", "w2v_destroy", "w2v_last_error", "w2v_model_upload", "w2v_model_download",
     "w2v_checkpoint_save", "w2v_checkpoint_load", "w2v_checkpoint_read_header", "w2v_sigmoid_tables", "w2v_submit_padded", "w2v_submit_csr", "w2v_sync", "w2v_sync_all",
     "w2v_set_delta_mode", "w2v_delta_download", "w2v_tables_upload", "w2v_set_schedule",
-    "w2v_corpus_upload", "w2v_train_device", "w2v_train_chunk", "w2v_get_stats", "w2v_heldout_loss",
+    "w2v_corpus_upload", "w2v_train_device", "w2v_train_epoch_graph", "w2v_train_chunk", "w2v_get_stats", "w2v_heldout_loss",
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
     "w2v_synth_corpus", "w2v_synth_cluster_of", "w2v_corpus_info", "w2v_corpus_download",
     "w2v_corpus_counts", "w2v_set_launch_shape", "w2v_get_launch_shape",
@@ -93,6 +93,7 @@ def lib() -> ctypes.CDLL:
             "w2v_set_schedule": [vp, ctypes.c_float, i32, u64],
             "w2v_corpus_upload": [vp, vp, i64, vp, i64],
             "w2v_train_device": [vp, i32, i32, i64, i64],
+            "w2v_train_epoch_graph": [vp, i32, i32, vp, i32],
             "w2v_train_chunk": [vp, i32, i32, vp, vp, i64, i64],
             "w2v_get_stats": [vp, ctypes.POINTER(Stats)],
             "w2v_heldout_loss": [vp, vp, vp, i64, ctypes.POINTER(ctypes.c_double),
@@ -260,6 +261,12 @@ class Trainer:
 
     def train_device(self, epoch, sent_begin, sent_end, stream=0):
         self._chk(lib().w2v_train_device(self.h, stream, epoch, sent_begin, sent_end))
+
+    def train_epoch_graph(self, epoch, bounds, stream=0):
+        """One epoch (or part) as one hipGraph launch: steps over sentences
+        [bounds[i], bounds[i+1])."""
+        b = np.ascontiguousarray(bounds, np.int64)
+        self._chk(lib().w2v_train_epoch_graph(self.h, stream, epoch, _ptr(b), len(b) - 1))
 
     def train_chunk(self, epoch, tokens, offsets, first_token, stream=0):
         t = np.ascontiguousarray(tokens, np.int32)

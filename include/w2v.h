@@ -160,6 +160,18 @@ int w2v_corpus_upload(w2v_ctx* ctx, const int32_t* tokens, int64_t n_tokens,
 int w2v_train_device(w2v_ctx* ctx, int32_t stream, int32_t epoch, int64_t sent_begin,
                      int64_t sent_end);
 
+/* One epoch of the device corpus as a single hipGraph launch. The epoch is split into
+ * n_steps sentence ranges [bounds[i], bounds[i+1]); each step is the launch sequence of
+ * w2v_train_device (subsampling flags, their scan, kept tokens and offsets, work-queue
+ * reset, persistent update kernel), recorded by stream capture with every buffer sized
+ * beforehand, so the graph holds kernel nodes only (any other node is rejected). The
+ * executable is cached per stream for the same bounds and refreshed for the epoch at
+ * each call. Results equal w2v_train_device over the same steps. Timing: one event pair
+ * around the whole graph, counted as one launch per non-empty step. No counterpart in the reference,
+ * which launches per batch from host threads (driver.cu:286-293). */
+int w2v_train_epoch_graph(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int64_t* bounds,
+                          int32_t n_steps);
+
 /* Streams one chunk of corpus sentences from host memory and trains it with device-side
  * sampling: sentence s of the chunk holds tokens[offsets[s]..offsets[s+1]) (offsets
  * relative, offsets[0] = 0), and tokens[0] sits at corpus position first_token (the key of

@@ -1,0 +1,96 @@
+"""hipGraph-captured epochs (w2v_train_epoch_graph) against the eager device path.
+
+Deterministic mode (one wave, ordered flushes): every graph launch, including replays of
+the cached executable for a new epoch, leaves the model bit-identical to the same steps
+launched one by one with w2v_train_device. Between graph launches the program runs its
+own device work (held-out loss, equal up to the FP64 summation order of its atomics) and
+synchronises, as a caller would. Hogwild mode: same words trained, finite model, held-out
+loss within 0.5%.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make(w2v, kernel, storage, mode, V=1500, d=64, W=3, K=4, n_sent=1200, seed=3):
+    tr = w2v.Trainer(V, d, W, K, msl=300, storage=storage, kernel=kernel, sigmoid="table", mode=mode,
+                     seed=seed)
+    cfg = w2v.synth_config(clusters=16, len_median=22, len_max=300, seed=seed)
+    tr.synth_corpus(cfg, n_sent)
+    nt, ns = tr.corpus_info()
+    cn = tr.corpus_counts()
+    cn[0] = ns
+    tr.tables_upload(w2v.unigram_table(cn, 1_000_000), w2v.subsample_probs(cn, 1e-3, nt))
+    tr.set_schedule(0.025, 3, nt)
+    tr.init_model()
+    return tr, ns
+
+
+def heldout(w2v, tr, n=200):
+    cfg = w2v.synth_config(clusters=16, len_median=22, len_max=300, seed=99)
+    h = w2v.Trainer(1500, 8, 1, 1)  # only to generate held-out sentences
+    h.synth_corpus(cfg, n)
+    tok, off = h.corpus_download(0, n)
+    h.close()
+    return tr.heldout_loss(tok, off)
+
+
+@pytest.mark.parametrize("kernel,storage", [("plain", "fp32"), ("mfma", "bf16")])
+def test_epoch_graph_equals_eager_deterministic(w2vmod, kernel, storage):
+    ea, ns = make(w2vmod, kernel, storage, "deterministic")
+    gb, _ = make(w2vmod, kernel, storage, "deterministic")
+    bounds = np.array([0, ns // 5, ns // 2, ns // 2, (4 * ns) // 5, ns], np.int64)  # one empty step
+    for call, epoch in enumerate([0, 1, 1]):
+        for i in range(len(bounds) - 1):
+            ea.train_device(epoch, int(bounds[i]), int(bounds[i + 1]))
+        gb.train_epoch_graph(epoch, bounds)
+        a0, a1 = ea.download()
+        b0, b1 = gb.download()
+        print(f"{kernel} call {call} epoch {epoch}: max |diff| "
+              f"{max(np.abs(a0 - b0).max(), np.abs(a1 - b1).max()):.3e}", flush=True)
+        assert np.array_equal(a0, b0) and np.array_equal(a1, b1), f"call {call} (epoch {epoch})"
+        # eager device work and synchronisation between graph launches
+        la, pa = heldout(w2vmod, ea)
+        lb, pb = heldout(w2vmod, gb)
+        # same model; the FP64 atomic accumulation order of the loss kernel varies
+        assert pa == pb and abs(la - lb) <= 1e-12 * abs(la)
+    sa, sb = ea.stats(), gb.stats()
+    assert sa["words_read"] == sb["words_read"] and sa["words_trained"] == sb["words_trained"]
+    assert sb["launches"] == sa["launches"]
+    ea.close()
+    gb.close()
+
+
+@pytest.mark.parametrize("kernel,storage", [("plain", "fp32"), ("mfma", "bf16")])
+def test_epoch_graph_hogwild(w2vmod, kernel, storage):
+    ea, ns = make(w2vmod, kernel, storage, "hogwild", n_sent=20000, d=128)
+    gb, _ = make(w2vmod, kernel, storage, "hogwild", n_sent=20000, d=128)
+    for t in (ea, gb):
+        t.set_launch_shape(1, 64)  # bounded concurrency on a small corpus
+    bounds = np.linspace(0, ns, 9).astype(np.int64)
+    for epoch in range(3):
+        for i in range(len(bounds) - 1):
+            ea.train_device(epoch, int(bounds[i]), int(bounds[i + 1]))
+        gb.train_epoch_graph(epoch, bounds)
+    la, _ = heldout(w2vmod, ea)
+    lb, _ = heldout(w2vmod, gb)
+    b0, b1 = gb.download()
+    sa, sb = ea.stats(), gb.stats()
+    print(f"{kernel} hogwild held-out loss eager {la:.5f} graph {lb:.5f}", flush=True)
+    assert np.isfinite(b0).all() and np.isfinite(b1).all()
+    assert sa["words_read"] == sb["words_read"]
+    assert abs(sa["words_trained"] - sb["words_trained"]) == 0
+    assert abs(la - lb) <= 0.005 * la
+    ea.close()
+    gb.close()
+
+
+def test_epoch_graph_rejects_bad_bounds(w2vmod):
+    tr, ns = make(w2vmod, "plain", "fp32", "hogwild", n_sent=100)
+    for bad in ([0, ns + 1], [5, 2], [-1, 3]):
+        with pytest.raises(w2vmod.W2VError):
+            tr.train_epoch_graph(0, np.array(bad, np.int64))
+    with pytest.raises(w2vmod.W2VError):
+        tr.train_epoch_graph(5, np.array([0, ns], np.int64))  # epoch outside the schedule
+    tr.close()
