@@ -23,6 +23,13 @@ bool mfma_supported(int d, int W, int K);
 hipError_t occupancy_plain(int P, bool delta, int waves_per_block, size_t lds, int* blocks_per_cu);
 size_t plain_lds_per_wave(int W, int P);
 hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
+int64_t ba_edge_count(int64_t n, int32_t m);
+int64_t ba_graph(int64_t n, int32_t m, uint64_t seed, int64_t* off, int32_t* adj);
+hipError_t launch_walk_lengths(int64_t n_walks, uint64_t ms, int32_t L, int32_t lmin, uint64_t thr,
+                               int64_t* len, hipStream_t st);
+hipError_t launch_walk_tokens(int64_t n_walks, int64_t n_nodes, uint64_t ms, const int64_t* goff,
+                              const int32_t* adj, const int64_t* soff, int32_t base, int32_t* out,
+                              hipStream_t st);
 size_t compact_temp_bytes(int64_t n);
 hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t sb, int64_t se,
                           const float* subs_p, uint64_t mixed_seed, int32_t epoch,
@@ -1237,6 +1244,97 @@ int w2v_corpus_download(w2v_ctx* c, int64_t sb, int64_t se, int32_t* tokens, int
   for (int64_t s = sb; s <= se; s++) offsets[s - sb] = c->h_off[s] - t0;
   if (t1 > t0 && tokens)
     HIP_TRY(c, hipMemcpy(tokens, c->d_corpus + t0, (size_t)(t1 - t0) * 4, hipMemcpyDeviceToHost));
+  return W2V_OK;
+}
+
+int w2v_graph_ba_edges(int64_t n, int32_t m, int64_t* edges) {
+  if (!edges) return W2V_ERR_ARG;
+  const int64_t E = ba_edge_count(n, m);
+  if (E < 0) return W2V_ERR_ARG;
+  *edges = E;
+  return W2V_OK;
+}
+
+int w2v_graph_ba(int64_t n, int32_t m, uint64_t seed, int64_t* offsets, int32_t* neighbors) {
+  if (!offsets || !neighbors || n > INT32_MAX) return W2V_ERR_ARG;
+  return ba_graph(n, m, seed, offsets, neighbors) < 0 ? W2V_ERR_ARG : W2V_OK;
+}
+
+int w2v_synth_walks(w2v_ctx* c, const int64_t* offsets, const int32_t* neighbors, int64_t n_nodes,
+                    const w2v_walks* cfg) {
+  if (!c || !offsets || !cfg || n_nodes < 1) return W2V_ERR_ARG;
+  if (cfg->token_base < 0 || n_nodes + cfg->token_base > c->p.vocab_size)
+    return fail(c, W2V_ERR_ARG, "%lld nodes from token %d exceed the vocabulary (%lld)", (long long)n_nodes,
+                cfg->token_base, (long long)c->p.vocab_size);
+  if (cfg->walks_per_node < 1 || cfg->walk_length < 1 || cfg->trunc_min < 1 ||
+      cfg->trunc_min > cfg->walk_length || !(cfg->trunc_frac >= 0.0 && cfg->trunc_frac <= 1.0))
+    return fail(c, W2V_ERR_ARG, "invalid walk configuration");
+  const int64_t n_walks = n_nodes * (int64_t)cfg->walks_per_node;
+  if (n_walks >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "at most 2^31-2 walks");
+  if (offsets[0] != 0) return fail(c, W2V_ERR_ARG, "graph offsets[0] must be 0");
+  for (int64_t i = 0; i < n_nodes; i++)
+    if (offsets[i + 1] < offsets[i]) return fail(c, W2V_ERR_ARG, "graph offsets must be non-decreasing");
+  const int64_t n_adj = offsets[n_nodes];
+  if (n_adj > 0 && !neighbors) return fail(c, W2V_ERR_ARG, "neighbors required");
+  for (int64_t i = 0; i < n_adj; i++)
+    if (neighbors[i] < 0 || neighbors[i] >= n_nodes) return fail(c, W2V_ERR_ARG, "neighbor %lld out of range", (long long)i);
+  int rc = w2v_sync_all(c);
+  if (rc) return rc;
+  hipStream_t st = c->ss[0].st;
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  if (c->d_corpus) HIP_TRY(c, hipFree(c->d_corpus));
+  if (c->d_off) HIP_TRY(c, hipFree(c->d_off));
+  c->d_corpus = nullptr;
+  c->d_off = nullptr;
+  c->n_tokens = c->n_sent = 0;
+  c->h_off.clear();
+  DevBuf goff, gadj, len, tmp;
+  auto cleanup = [&]() {
+    for (DevBuf* b : {&goff, &gadj, &len, &tmp})
+      if (b->p) (void)hipFree(b->p);
+  };
+  if ((rc = ensure_dev(c, goff, (size_t)(n_nodes + 1) * 8)) || (rc = ensure_dev(c, gadj, (size_t)(n_adj > 0 ? n_adj : 1) * 4)) ||
+      (rc = ensure_dev(c, len, (size_t)(n_walks + 1) * 8))) {
+    cleanup();
+    return rc;
+  }
+  const uint64_t ms = mix64(cfg->seed);
+  const uint64_t thr = (uint64_t)(cfg->trunc_frac * 4294967296.0);
+  hipError_t e = hipMemcpyAsync(goff.p, offsets, (size_t)(n_nodes + 1) * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && n_adj) e = hipMemcpyAsync(gadj.p, neighbors, (size_t)n_adj * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_walk_lengths(n_walks, ms, cfg->walk_length, cfg->trunc_min, thr, (int64_t*)len.p, st);
+  if (e == hipSuccess && hipMalloc(&c->d_off, (size_t)(n_walks + 1) * 8) != hipSuccess) e = hipErrorOutOfMemory;
+  size_t tb = 0;
+  if (e == hipSuccess) e = scan_lengths((const int64_t*)len.p, c->d_off, n_walks, nullptr, &tb, st);
+  if (e == hipSuccess) {
+    if ((rc = ensure_dev(c, tmp, tb))) {
+      cleanup();
+      return rc;
+    }
+    e = scan_lengths((const int64_t*)len.p, c->d_off, n_walks, tmp.p, &tb, st);
+  }
+  if (e == hipSuccess) {
+    c->h_off.resize(n_walks + 1);
+    e = hipMemcpyAsync(c->h_off.data(), c->d_off, (size_t)(n_walks + 1) * 8, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) {
+    const int64_t nt = c->h_off[n_walks];
+    if (hipMalloc(&c->d_corpus, (size_t)(nt > 0 ? nt : 1) * 4) != hipSuccess) e = hipErrorOutOfMemory;
+    else
+      e = launch_walk_tokens(n_walks, n_nodes, ms, (const int64_t*)goff.p, (const int32_t*)gadj.p, c->d_off,
+                             cfg->token_base, c->d_corpus, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) {
+      c->n_tokens = nt;
+      c->n_sent = n_walks;
+    }
+  }
+  cleanup();
+  if (e != hipSuccess) {
+    c->h_off.clear();
+    return fail(c, W2V_ERR_HIP, "walk corpus generation failed: %s", hipGetErrorString(e));
+  }
   return W2V_OK;
 }
 

@@ -29,7 +29,8 @@ EXPORTS = [
     "w2v_corpus_upload", "w2v_train_device", "w2v_train_epoch_graph", "w2v_train_chunk", "w2v_get_stats", "w2v_heldout_loss",
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
     "w2v_synth_corpus", "w2v_synth_cluster_of", "w2v_corpus_info", "w2v_corpus_download",
-    "w2v_corpus_counts", "w2v_set_launch_shape", "w2v_get_launch_shape",
+    "w2v_corpus_counts", "w2v_graph_ba_edges", "w2v_graph_ba", "w2v_synth_walks",
+    "w2v_set_launch_shape", "w2v_get_launch_shape",
 ]
 
 
@@ -42,6 +43,13 @@ class Params(ctypes.Structure):
         ("device", ctypes.c_int32), ("streams", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
     ]
+
+
+class Walks(ctypes.Structure):
+    _fields_ = [("walks_per_node", ctypes.c_int32), ("walk_length", ctypes.c_int32),
+                ("trunc_min", ctypes.c_int32), ("token_base", ctypes.c_int32),
+                ("trunc_frac", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
 
 
 class Synth(ctypes.Structure):
@@ -107,6 +115,9 @@ def lib() -> ctypes.CDLL:
             "w2v_corpus_info": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
             "w2v_corpus_download": [vp, i64, i64, vp, vp],
             "w2v_corpus_counts": [vp, vp],
+            "w2v_graph_ba_edges": [i64, i32, ctypes.POINTER(i64)],
+            "w2v_graph_ba": [i64, i32, u64, vp, vp],
+            "w2v_synth_walks": [vp, vp, vp, i64, ctypes.POINTER(Walks)],
             "w2v_set_launch_shape": [vp, i32, i32],
             "w2v_get_launch_shape": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)],
         }
@@ -137,6 +148,21 @@ def checkpoint_header(path):
     if rc:
         raise W2VError(rc, lib().w2v_last_error(None).decode())
     return {k: getattr(m, k) for k, _ in Checkpoint._fields_}
+
+
+def graph_ba(n_nodes, m, seed):
+    """Barabasi-Albert graph from the library's host generator: CSR (offsets int64[n+1],
+    neighbors int32[2E])."""
+    E = ctypes.c_int64()
+    rc = lib().w2v_graph_ba_edges(int(n_nodes), int(m), ctypes.byref(E))
+    if rc:
+        raise W2VError(rc, "w2v_graph_ba_edges")
+    off = np.zeros(int(n_nodes) + 1, np.int64)
+    adj = np.zeros(2 * E.value, np.int32)
+    rc = lib().w2v_graph_ba(int(n_nodes), int(m), int(seed), _ptr(off), _ptr(adj))
+    if rc:
+        raise W2VError(rc, "w2v_graph_ba")
+    return off, adj
 
 
 def unigram_table(counts, n=100_000_000, corrected=False):
@@ -292,6 +318,15 @@ class Trainer:
 
     def synth_corpus(self, cfg, n_sentences, first=0):
         self._chk(lib().w2v_synth_corpus(self.h, ctypes.byref(cfg), int(first), int(n_sentences)))
+
+    def synth_walks(self, offsets, neighbors, walks_per_node=10, walk_length=80, trunc_min=5,
+                    trunc_frac=0.3, seed=1, token_base=1):
+        """Device corpus of random walks over a CSR graph; node u is token token_base + u
+        (1 keeps id 0 for </s>, as in the text corpora)."""
+        o = np.ascontiguousarray(offsets, np.int64)
+        a = np.ascontiguousarray(neighbors, np.int32)
+        cfg = Walks(walks_per_node, walk_length, trunc_min, int(token_base), float(trunc_frac), int(seed))
+        self._chk(lib().w2v_synth_walks(self.h, _ptr(o), _ptr(a), len(o) - 1, ctypes.byref(cfg)))
 
     def corpus_info(self):
         nt, ns = ctypes.c_int64(), ctypes.c_int64()

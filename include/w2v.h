@@ -249,6 +249,32 @@ int w2v_corpus_download(w2v_ctx* ctx, int64_t sent_begin, int64_t sent_end, int3
 /* Occurrence count of every word id in the device corpus (counts has V entries). */
 int w2v_corpus_counts(w2v_ctx* ctx, uint64_t* counts);
 
+/* ---- graph-walk corpus (DeepWalk-style sentences over a power-law graph) ---------
+ * No counterpart in the reference, which trains on text corpora only; these build the
+ * uneven-length walk corpus used to exercise the persistent, graph-captured batching.
+ * Barabasi-Albert preferential attachment on the host: nodes 0..m form a complete
+ * graph, every later node attaches to m distinct earlier nodes drawn in proportion to
+ * their degree (keyed stream, seed). CSR with both directions: offsets[n+1],
+ * neighbors[2E]; w2v_graph_ba_edges gives E. */
+int w2v_graph_ba_edges(int64_t n_nodes, int32_t m, int64_t* edges);
+int w2v_graph_ba(int64_t n_nodes, int32_t m, uint64_t seed, int64_t* offsets, int32_t* neighbors);
+
+typedef struct w2v_walks {
+  int32_t walks_per_node; /* walks started from every node */
+  int32_t walk_length;    /* nodes per walk */
+  int32_t trunc_min;      /* shortest truncated walk */
+  int32_t token_base;     /* token id of node 0 (1 keeps id 0 for </s>, as text corpora) */
+  double trunc_frac;      /* share of walks truncated to a length uniform in [trunc_min, walk_length] */
+  uint64_t seed;
+} w2v_walks;
+
+/* Replaces the device corpus with uniform random walks over a CSR graph (node ids are
+ * token_base + node id, token_base + n_nodes <= vocab_size): walk w = r*n + u starts at node u and each step
+ * moves to a uniformly drawn neighbor, all draws from keyed streams of cfg->seed, one
+ * device thread per walk. Sentences are the walks in that order. */
+int w2v_synth_walks(w2v_ctx* ctx, const int64_t* offsets, const int32_t* neighbors, int64_t n_nodes,
+                    const w2v_walks* cfg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -736,3 +736,82 @@ double orc_heldout_loss(const float* syn0, const float* syn1, int32_t d, int32_t
   if (pairs_out) *pairs_out = pairs;
   return pairs ? sum / pairs : 0.0;
 }
+
+/* ---- graph-walk corpus ---------------------------------------------------------
+ * Test infrastructure: restates the generator specified in sgns_oracle.h (the
+ * reference itself has no synthetic graph; its evaluation uses text corpora). */
+int64_t orc_ba_edges(int64_t n, int32_t m) {
+  if (m < 1 || n <= m) return -1;
+  return (int64_t)m * (m + 1) / 2 + (n - m - 1) * (int64_t)m;
+}
+
+int64_t orc_ba_graph(int64_t n, int32_t m, uint64_t seed, int64_t* offsets, int32_t* neighbors) {
+  const int64_t E = orc_ba_edges(n, m);
+  if (E < 0 || n > INT32_MAX) return -1;
+  int32_t* ea = (int32_t*)malloc((size_t)E * 4);
+  int32_t* eb = (int32_t*)malloc((size_t)E * 4);
+  int32_t* rep = (int32_t*)malloc((size_t)E * 8);
+  int32_t* chosen = (int32_t*)malloc((size_t)m * 4);
+  int64_t e = 0, nr = 0;
+  for (int32_t i = 0; i <= m; i++)
+    for (int32_t j = i + 1; j <= m; j++) {
+      ea[e] = i; eb[e] = j; e++;
+      rep[nr++] = i; rep[nr++] = j;
+    }
+  for (int64_t v = m + 1; v < n; v++) {
+    const uint64_t len = (uint64_t)nr;
+    int32_t got = 0;
+    uint64_t attempt = 0;
+    while (got < m) {
+      const uint64_t x = orc_key(seed, (0x42414752ull << 32) ^ (uint64_t)v, attempt++);
+      const int32_t t = rep[((x >> 32) * len) >> 32];
+      int dup = 0;
+      for (int32_t k = 0; k < got; k++) dup |= chosen[k] == t;
+      if (!dup) chosen[got++] = t;
+    }
+    for (int32_t k = 0; k < m; k++) {
+      ea[e] = (int32_t)v; eb[e] = chosen[k]; e++;
+      rep[nr++] = (int32_t)v; rep[nr++] = chosen[k];
+    }
+  }
+  for (int64_t i = 0; i <= n; i++) offsets[i] = 0;
+  for (int64_t k = 0; k < E; k++) { offsets[ea[k] + 1]++; offsets[eb[k] + 1]++; }
+  for (int64_t i = 0; i < n; i++) offsets[i + 1] += offsets[i];
+  int64_t* cur = (int64_t*)malloc((size_t)n * 8);
+  for (int64_t i = 0; i < n; i++) cur[i] = offsets[i];
+  for (int64_t k = 0; k < E; k++) {
+    neighbors[cur[ea[k]]++] = eb[k];
+    neighbors[cur[eb[k]]++] = ea[k];
+  }
+  free(cur); free(ea); free(eb); free(rep); free(chosen);
+  return E;
+}
+
+int64_t orc_walks(const int64_t* offsets, const int32_t* neighbors, int64_t n, int32_t walks_per_node,
+                  int32_t walk_length, int32_t trunc_min, uint64_t trunc_thr, uint64_t seed,
+                  int64_t* sent_off, int32_t* tokens) {
+  const int64_t nw = n * (int64_t)walks_per_node;
+  const uint64_t salt = 0x57414C4Bull << 32;
+  sent_off[0] = 0;
+  for (int64_t w = 0; w < nw; w++) {
+    const uint64_t x0 = orc_key(seed, salt ^ (uint64_t)w, 0);
+    int64_t len = walk_length;
+    if ((x0 >> 32) < trunc_thr)
+      len = trunc_min + (int64_t)(((x0 & 0xFFFFFFFFull) * (uint64_t)(walk_length - trunc_min + 1)) >> 32);
+    sent_off[w + 1] = sent_off[w] + len;
+  }
+  for (int64_t w = 0; w < nw; w++) {
+    int32_t cur = (int32_t)(w % n);
+    const int64_t b = sent_off[w], len = sent_off[w + 1] - b;
+    tokens[b] = cur;
+    for (int64_t st = 1; st < len; st++) {
+      const int64_t o = offsets[cur], deg = offsets[cur + 1] - o;
+      if (deg > 0) {
+        const uint64_t x = orc_key(seed, salt ^ (uint64_t)w, (uint64_t)st);
+        cur = neighbors[o + (int64_t)(((x >> 32) * (uint64_t)deg) >> 32)];
+      }
+      tokens[b + st] = cur;
+    }
+  }
+  return sent_off[nw];
+}

@@ -23,6 +23,11 @@
  *   orc_assemble_*       hand-traced small corpora (counts, sentence breaks).
  *   orc_synth_*          statistical properties (Zipf slope, stay probability).
  *   orc_heldout_loss     closed form on a 1-pair corpus.
+ *   orc_ba_graph         edge count closed form, degree sum, minimum degree m, no
+ *                        self loops or repeated edges, preferential attachment (the
+ *                        degree tail of a BA graph, P(k) ~ k^-3).
+ *   orc_walks            every step follows an edge, each node starts exactly R walks,
+ *                        truncated share and length distribution.
  */
 #ifndef SGNS_ORACLE_H
 #define SGNS_ORACLE_H
@@ -124,6 +129,24 @@ int32_t orc_synth_length(const orc_synth_cfg* c, const uint64_t* lthr, int64_t s
 void orc_synth_sentence(const orc_synth_cfg* c, const uint64_t* gthr, const int32_t* members,
                         const int64_t* cstart, const uint64_t* cthr, int64_t s, int32_t len,
                         int32_t* out);
+
+/* ---- graph-walk corpus (DeepWalk-style sentences over a Barabasi-Albert graph) ----
+ * orc_ba_graph: nodes 0..m form a complete graph; every later node v draws m distinct
+ * targets, each uniform over the list of edge endpoints as it stands before v's edges
+ * (preferential attachment), draw i from orc_key(seed, (0x42414752<<32) ^ v, i); edges in
+ * creation order, adjacency lists (CSR, both directions) in edge order. Returns the
+ * number of undirected edges, or -1 for invalid (n, m). offsets: n+1, neighbors: 2E.
+ * orc_walks: walk w = r*n + u (r < walks_per_node) starts at node u; with x0 =
+ * orc_key(seed, (0x57414C4B<<32) ^ w, 0) it is truncated when (x0>>32) < trunc_thr to
+ * trunc_min + (((x0 & 0xFFFFFFFF) * (walk_length - trunc_min + 1)) >> 32) nodes, else has
+ * walk_length nodes; step s moves to neighbor ((orc_key(.., w, s)>>32) * deg) >> 32 of
+ * the current node (a node without neighbors repeats). Writes sent_off[n*R+1] and the
+ * tokens; returns the token count. */
+int64_t orc_ba_edges(int64_t n, int32_t m);
+int64_t orc_ba_graph(int64_t n, int32_t m, uint64_t seed, int64_t* offsets, int32_t* neighbors);
+int64_t orc_walks(const int64_t* offsets, const int32_t* neighbors, int64_t n, int32_t walks_per_node,
+                  int32_t walk_length, int32_t trunc_min, uint64_t trunc_thr, uint64_t seed,
+                  int64_t* sent_off, int32_t* tokens);
 
 /* ---- held-out SGNS loss (FP64) ----
  * Mean over (context, centre) pairs of  -log s(u_c.v_w) - sum_k log s(-u_c.v_nk),

@@ -78,6 +78,12 @@ def lib():
         L.orc_synth_tables.restype = ctypes.c_int
         L.orc_synth_length.argtypes = [ctypes.POINTER(SynthCfg), vp, i64]
         L.orc_synth_length.restype = i32
+        L.orc_ba_edges.argtypes = [i64, i32]
+        L.orc_ba_edges.restype = i64
+        L.orc_ba_graph.argtypes = [i64, i32, u64, vp, vp]
+        L.orc_ba_graph.restype = i64
+        L.orc_walks.argtypes = [vp, vp, i64, i32, i32, i32, u64, u64, vp, vp]
+        L.orc_walks.restype = i64
         L.orc_synth_sentence.argtypes = [ctypes.POINTER(SynthCfg), vp, vp, vp, vp, i64, i32, vp]
         L.orc_heldout_loss.argtypes = [vp, vp, i32, i32, i32, vp, vp, i64, vp, i64, u64,
                                        ctypes.POINTER(i64)]
@@ -224,3 +230,30 @@ def device_keep(seed, epoch, tokens, pos0, subs_p):
     subs_p = np.ascontiguousarray(subs_p, np.float32)
     return np.array([lib().orc_device_keep(seed, epoch, pos0 + i, int(w), P(subs_p))
                      for i, w in enumerate(tokens)], bool)
+
+
+def ba_graph(n, m, seed):
+    """Barabasi-Albert graph as CSR (offsets[n+1] int64, neighbors[2E] int32)."""
+    L = lib()
+    E = L.orc_ba_edges(n, m)
+    assert E >= 0
+    off = np.zeros(n + 1, np.int64)
+    adj = np.zeros(2 * E, np.int32)
+    assert L.orc_ba_graph(n, m, seed, P(off), P(adj)) == E
+    return off, adj
+
+
+def trunc_threshold(frac):
+    """Integer threshold of the walk truncation test (x >> 32) < thr."""
+    return int(min(max(frac, 0.0), 1.0) * 4294967296.0)
+
+
+def walks(off, adj, walks_per_node, walk_length, trunc_min, trunc_frac, seed):
+    """Random-walk sentences over a CSR graph: (tokens int32, sentence offsets int64)."""
+    n = len(off) - 1
+    nw = n * walks_per_node
+    sent = np.zeros(nw + 1, np.int64)
+    tok = np.zeros(nw * walk_length, np.int32)
+    nt = lib().orc_walks(P(off), P(adj), n, walks_per_node, walk_length, trunc_min,
+                         trunc_threshold(trunc_frac), seed, P(sent), P(tok))
+    return tok[:nt].copy(), sent

@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "w2v.h")
 def declared_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return re.findall(r"^\s*(?:int|int32_t|const char\*)\s+(w2v_\w+)\s*\(", txt, flags=re.M)
+    return re.findall(r"^\s*(?:int|int32_t|int64_t|const char\*)\s+(w2v_\w+)\s*\(", txt, flags=re.M)
 
 
 def test_header_declarations_exported(w2vmod):

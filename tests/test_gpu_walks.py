@@ -1,0 +1,123 @@
+"""Graph-walk corpus on the device, and training on it (the uneven-length batching case).
+
+* The device walk generator reproduces the oracle's walks token for token.
+* Training on walks over a power-law graph (Barabasi-Albert, m=5; 10 walks of 80 nodes
+  from every node, 30% truncated to a length uniform in 5..80; window 10, 5 negatives,
+  sample 1e-4, one epoch) through the persistent kernel fed by the device work queue,
+  each epoch captured as one hipGraph of four steps: held-out SGNS loss of both GPU
+  kernels within 1% of the CPU reference trainer on the same sentences, and link
+  prediction (ROC AUC of the cosine of graph neighbours against random node pairs)
+  within 0.02 of it. Bounded concurrency (256 sentences in flight) on this small corpus,
+  as in the text-corpus quality test.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N_NODES, M, R, L, LMIN, FRAC = 20000, 5, 10, 80, 5, 0.3
+V, D, W, K = N_NODES + 1, 64, 10, 5  # node u is token u + 1; id 0 stays </s>
+
+
+def test_device_walks_equal_oracle(orc, w2vmod):
+    off, adj = orc.ba_graph(5000, 5, 21)
+    tr = w2vmod.Trainer(5001, 16, 2, 2)
+    tr.synth_walks(off, adj, walks_per_node=3, walk_length=40, trunc_min=5, trunc_frac=0.3, seed=33,
+                   token_base=1)
+    nt, ns = tr.corpus_info()
+    tok, so = tr.corpus_download(0, ns)
+    tr.close()
+    otok, oso = orc.walks(off, adj, 3, 40, 5, 0.3, 33)
+    assert ns == 15000 and nt == len(otok)
+    assert np.array_equal(so, oso) and np.array_equal(tok, otok + 1)
+
+
+def test_walks_reject_bad_config(w2vmod):
+    off, adj = w2vmod.graph_ba(100, 2, 1)
+    tr = w2vmod.Trainer(101, 16, 2, 2)
+    for kw in (dict(trunc_min=0), dict(trunc_min=90), dict(trunc_frac=1.5), dict(walks_per_node=0),
+               dict(token_base=2)):
+        args = dict(walks_per_node=2, walk_length=80, trunc_min=5, trunc_frac=0.3, seed=1, token_base=1)
+        args.update(kw)
+        with pytest.raises(w2vmod.W2VError):
+            tr.synth_walks(off, adj, **args)
+    bad = adj.copy()
+    bad[0] = 100
+    with pytest.raises(w2vmod.W2VError):
+        tr.synth_walks(off, bad)
+    tr.close()
+
+
+def link_auc(syn0, off, adj, seed=0, n=20000):
+    """ROC AUC of cos(u, v) for graph edges against uniformly drawn node pairs."""
+    rng = np.random.default_rng(seed)
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    pick = rng.integers(0, len(adj), n)
+    pos = (src[pick], adj[pick])
+    neg = (rng.integers(0, len(off) - 1, n), rng.integers(0, len(off) - 1, n))
+    x = syn0[1:].astype(np.float64)
+    x /= np.linalg.norm(x, axis=1, keepdims=True) + 1e-12
+    sp = np.sum(x[pos[0]] * x[pos[1]], axis=1)
+    sn = np.sum(x[neg[0]] * x[neg[1]], axis=1)
+    from scipy.stats import mannwhitneyu
+    return float(mannwhitneyu(sp, sn).statistic / (n * n))
+
+
+@pytest.fixture(scope="module")
+def runs(orc, w2vmod):
+    off, adj = w2vmod.graph_ba(N_NODES, M, 5)
+    g = w2vmod.Trainer(V, D, W, K)
+    g.synth_walks(off, adj, R, L, LMIN, FRAC, seed=7)
+    T, ns = g.corpus_info()
+    tok, so = g.corpus_download(0, ns)
+    cn = g.corpus_counts()
+    g.synth_walks(off, adj, 1, L, LMIN, FRAC, seed=8)  # held-out walks
+    _, hn = g.corpus_info()
+    htok, hoff = g.corpus_download(0, hn)
+    g.close()
+    cn[0] = ns  # one </s> per walk line
+    cn = np.maximum(cn, 1)
+    uni = w2vmod.unigram_table(cn, 10_000_000)
+    subs = w2vmod.subsample_probs(cn, 1e-4, T)
+    out = {}
+    # CPU reference trainer on the same walks, one line each
+    lens = np.diff(so)
+    starts = so[:-1] + np.arange(ns)
+    stream = np.zeros(T + ns + 1, np.int32)
+    for s in range(ns):
+        stream[starts[s]:starts[s] + lens[s]] = tok[so[s]:so[s + 1]]
+    s0 = orc.init_syn0(V, D)
+    s1 = np.zeros((V, D), np.float32)
+    threads = min(16, len(os.sched_getaffinity(0)))
+    orc.train_hogwild(stream, starts.astype(np.int64), T, uni, subs, s0, s1, D, W, K, 1000, 1, threads,
+                      0.025, 1e-4)
+    out["cpu"] = (orc.heldout_loss(s0, s1, W, K, htok, hoff, uni, 77)[0], link_auc(s0, off, adj))
+    bounds = np.linspace(0, ns, 5).astype(np.int64)
+    for kernel, storage in (("plain", "fp32"), ("mfma", "bf16")):
+        t = w2vmod.Trainer(V, D, W, K, storage=storage, kernel=kernel, seed=3)
+        t.set_launch_shape(1, 256)
+        t.corpus_upload(tok, so)
+        t.tables_upload(uni, subs)
+        t.set_schedule(0.025, 1, T)
+        t.init_model()
+        t.train_epoch_graph(0, bounds)
+        g0, g1 = t.download()
+        st = t.stats()
+        t.close()
+        assert st["words_read"] == T
+        out[kernel] = (orc.heldout_loss(g0, g1, W, K, htok, hoff, uni, 77)[0], link_auc(g0, off, adj))
+    print(f"walk corpus: {ns} walks, {T} tokens; held-out loss / link AUC: {out}", flush=True)
+    return out
+
+
+@pytest.mark.parametrize("kernel", ["plain", "mfma"])
+def test_walks_heldout_loss_within_1pct_of_cpu(runs, kernel):
+    assert abs(runs[kernel][0] - runs["cpu"][0]) <= 0.01 * runs["cpu"][0]
+
+
+@pytest.mark.parametrize("kernel", ["plain", "mfma"])
+def test_walks_link_prediction_matches_cpu(runs, kernel):
+    assert runs["cpu"][1] > 0.8
+    assert abs(runs[kernel][1] - runs["cpu"][1]) <= 0.02
