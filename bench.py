@@ -189,6 +189,72 @@ def run_streamed(w2v, steps, warmup, host):
                 host_bytes_per_step=float(4 * words / max(steps, 1)))
 
 
+WALKS = dict(nodes=1_000_000, m=5, walks_per_node=10, walk_length=80, trunc_min=5, trunc_frac=0.3,
+             d=128, W=10, K=5, sample=1e-4, alpha=0.025, graph_seed=1, walk_seed=1, heldout_seed=2,
+             heldout_walks=20_000)
+
+
+def run_walks(w2v, steps, warmup):
+    """Graph-walk workload: a Barabasi-Albert graph (m=5) of 1M nodes, 10 uniform random
+    walks of 80 nodes from every node with 30% truncated to a length uniform in 5..80
+    (sentences of very uneven length), d=128, window 10, 5 negatives. One epoch in
+    `steps` equal sentence ranges, trained by each kernel twice: one launch sequence per
+    step (w2v_train_device), and the whole epoch as one captured hipGraph
+    (w2v_train_epoch_graph). Node u is token u+1 (id 0 is </s>)."""
+    c = WALKS
+    n, V = c["nodes"], c["nodes"] + 1
+    t = time.perf_counter()
+    off, adj = w2v.graph_ba(n, c["m"], c["graph_seed"])
+    log(f"[walks] BA graph: {n} nodes, {len(adj) // 2} edges ({time.perf_counter() - t:.1f}s)")
+    results = {}
+    for kernel, storage in variants(w2v):
+        for mode in ("steps", "graph"):
+            tr = w2v.Trainer(V, c["d"], c["W"], c["K"], msl=c["walk_length"], storage=storage, kernel=kernel,
+                             sigmoid="table", mode="hogwild", seed=1)
+            t = time.perf_counter()
+            tr.synth_walks(off, adj, c["walks_per_node"], c["walk_length"], c["trunc_min"], c["trunc_frac"],
+                           seed=c["walk_seed"])
+            T, ns = tr.corpus_info()
+            cn = tr.corpus_counts()
+            cn[0] = ns
+            cn = np.maximum(cn, 1)
+            log(f"[walks/{kernel}/{mode}] corpus: {ns} walks, {T} tokens ({time.perf_counter() - t:.1f}s)")
+            tr.tables_upload(w2v.unigram_table(cn, WL["unigram"]), w2v.subsample_probs(cn, c["sample"], T))
+            tr.set_schedule(c["alpha"], 2, T)
+            tr.init_model()
+            bounds = np.array([round(i * ns / steps) for i in range(steps + 1)], np.int64)
+            if warmup:  # a warm-up epoch (epoch 0 of the schedule), then the timed one
+                if mode == "graph":
+                    tr.train_epoch_graph(0, bounds)
+                else:
+                    for i in range(steps):
+                        tr.train_device(0, int(bounds[i]), int(bounds[i + 1]))
+                tr.sync()
+            st0 = tr.stats()
+            t0 = time.perf_counter()
+            if mode == "graph":
+                tr.train_epoch_graph(1, bounds)
+            else:
+                for i in range(steps):
+                    tr.train_device(1, int(bounds[i]), int(bounds[i + 1]))
+            tr.sync()
+            secs = time.perf_counter() - t0
+            st1 = tr.stats()
+            words = st1["words_read"] - st0["words_read"]
+            r = dict(words_per_sec=words / secs, seconds=secs, words=int(words),
+                     kept_words=int(st1["words_trained"] - st0["words_trained"]))
+            if mode == "graph":
+                tr.synth_walks(off, adj, 1, c["walk_length"], c["trunc_min"], c["trunc_frac"],
+                               seed=c["heldout_seed"])
+                htok, hoff = tr.corpus_download(0, c["heldout_walks"])
+                r["heldout_loss"], _ = tr.heldout_loss(htok, hoff)
+            tr.close()
+            log(f"[walks/{kernel}/{mode}] {r['words_per_sec'] / 1e6:.1f} M words/s "
+                + (f"held-out loss {r['heldout_loss']:.4f}" if "heldout_loss" in r else ""))
+            results[f"{kernel}_{mode}"] = r
+    return results
+
+
 def cpu_baseline(host, target_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
@@ -259,6 +325,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--streamed", action="store_true",
                     help="also time the MFMA path with the corpus streamed from host memory")
+    ap.add_argument("--workload", default="text", help="text (the headline workload) | walks")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to scale HBM byte counters")
     args = ap.parse_args()
@@ -271,6 +338,19 @@ def main():
         sys.exit(2)
     import w2v
 
+    if args.workload == "walks":
+        res = run_walks(w2v, max(args.steps, 1), args.warmup)
+        best = max(res.items(), key=lambda kv: kv[1]["words_per_sec"])
+        print(json.dumps({
+            "metric": "trained words/sec, graph-walk corpus (uneven sentences), d=128 w=10 neg=5",
+            "value": best[1]["words_per_sec"], "unit": "words/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16" if best[0].startswith("mfma") else "fp32", "data": "synthetic",
+            "config": {"workload": "Barabasi-Albert graph (1M nodes, m=5), 10 walks x 80 per node, "
+                                   "30% truncated to U[5,80], d=128 window 10 negative 5 sample 1e-4, 1 epoch",
+                       "best": best[0], "parallelism": "single-gpu"},
+            "variants": res}), flush=True)
+        return
     host = {}
     vs = variants(w2v)
     if args.variant != "all":

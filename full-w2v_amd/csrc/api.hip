@@ -92,6 +92,8 @@ struct w2v_ctx {
   int cfg_blocks[2] = {0, 0};
   int max_blocks[2] = {0, 0};  // device-filling grid at cfg_wpb
   int user_wpb = 0, user_blocks = 0;  // w2v_set_launch_shape override
+  int occ_bpc[5] = {0, 0, 0, 0, 0};   // Hogwild: resident workgroups per CU at 1..4 waves each
+  std::string arch;                   // device name (gcnArchName), the tuning cache key
   float* dsyn0 = nullptr;
   float* dsyn1 = nullptr;
   int delta_mode = 0;
@@ -324,6 +326,7 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
   if (hipGetDeviceProperties(&prop, p.device) != hipSuccess)
     return bail(fail(c, W2V_ERR_HIP, "hipGetDeviceProperties failed"));
   c->num_cu = prop.multiProcessorCount;
+  c->arch = prop.gcnArchName;
 
   if (p.kernel == W2V_KERNEL_PLAIN && plain_lds_per_wave(p.window, c->P) > 160 * 1024)
     return bail(fail(c, W2V_ERR_UNSUPPORTED,
@@ -364,6 +367,7 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
       hipError_t oe = mf ? occupancy_mfma(c->P, p.negative, p.window, dm != 0, wpb, lw * wpb, &bpc)
                          : occupancy_plain(c->P, dm != 0, wpb, lw * wpb, &bpc);
       if (oe != hipSuccess || bpc < 1) bpc = 1;
+      if (dm == 0) c->occ_bpc[wpb] = bpc;
       if (bpc * wpb > best_waves) {
         best_waves = bpc * wpb;
         best_wpb = wpb;
@@ -981,6 +985,127 @@ int w2v_train_chunk(w2v_ctx* c, int32_t stream, int32_t epoch, const int32_t* to
   a.n_sent = ns;
   c->words_read += (uint64_t)n;
   return launch_update(c, s, a);
+}
+
+int w2v_autotune(w2v_ctx* c, int32_t epoch, int64_t sb, int64_t se, int64_t max_in_flight,
+                 const char* cache_path, w2v_tuning* out) {
+  int rc = check_device_training(c, 0, epoch);
+  if (rc) return rc;
+  if (!out) return fail(c, W2V_ERR_ARG, "out required");
+  if (c->p.mode != W2V_MODE_HOGWILD || c->delta_mode)
+    return fail(c, W2V_ERR_STATE, "launch-shape tuning applies to Hogwild runs without delta buffers");
+  if (sb < 0 || se > c->n_sent || sb >= se) return fail(c, W2V_ERR_ARG, "sentence range out of bounds");
+  const int64_t n = c->h_off[se] - c->h_off[sb];
+  if (n < 1 || n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "tuning range must hold 1..2^31-2 tokens");
+  memset(out, 0, sizeof(*out));
+  char key[256];
+  snprintf(key, sizeof(key), "%s %s %s d=%d W=%d K=%d cap=%lld", c->arch.c_str(),
+           c->p.kernel == W2V_KERNEL_MFMA ? "mfma" : "plain", c->p.storage == W2V_STORAGE_BF16 ? "bf16" : "fp32",
+           c->p.layer_size, c->p.window, c->p.negative, (long long)max_in_flight);
+  if (cache_path) {
+    if (FILE* f = fopen(cache_path, "r")) {
+      char line[512];
+      const size_t kl = strlen(key);
+      while (fgets(line, sizeof(line), f)) {
+        int w = 0, b = 0;
+        double wps = 0.0;
+        if (strncmp(line, key, kl) == 0 && line[kl] == '|' &&
+            sscanf(line + kl + 1, "%d %d %lf", &w, &b, &wps) == 3 && w >= 1 && w <= 4 && b >= 1) {
+          fclose(f);
+          c->user_wpb = w;
+          c->user_blocks = b;
+          out->waves_per_block = w;
+          out->blocks = b;
+          out->words_per_sec = wps;
+          out->cached = 1;
+          return W2V_OK;
+        }
+      }
+      fclose(f);
+    }
+  }
+  if ((rc = w2v_sync_all(c))) return rc;
+  StreamState& s = c->ss[0];
+  HIP_TRY(c, hipSetDevice(c->p.device));
+  if ((rc = ensure_step_buffers(c, s, n, se - sb))) return rc;
+  // snapshot of the model and counters, restored after every trial and at the end
+  const bool bf = c->p.storage == W2V_STORAGE_BF16;
+  const size_t rows = (size_t)c->p.vocab_size * c->P * (bf ? 2 : 4);
+  void* m0 = bf ? (void*)c->bsyn0 : (void*)c->syn0;
+  void* m1 = bf ? (void*)c->bsyn1 : (void*)c->syn1;
+  void *k0 = nullptr, *k1 = nullptr;
+  if (hipMalloc(&k0, rows) != hipSuccess || hipMalloc(&k1, rows) != hipSuccess) {
+    (void)hipFree(k0);
+    return fail(c, W2V_ERR_NOMEM, "tuning snapshot of 2 x %zu bytes failed", rows);
+  }
+  unsigned long long cnt[4];
+  const uint64_t words0 = c->words_read, launches0 = c->launches, slaunch0 = s.launches;
+  const double sms0 = s.ms;
+  hipError_t e = hipMemcpyAsync(k0, m0, rows, hipMemcpyDeviceToDevice, s.st);
+  if (e == hipSuccess) e = hipMemcpyAsync(k1, m1, rows, hipMemcpyDeviceToDevice, s.st);
+  if (e == hipSuccess) e = hipMemcpyAsync(cnt, c->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, s.st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s.st);
+  const int uw0 = c->user_wpb, ub0 = c->user_blocks;
+  int best_w = 0, best_b = 0, tried = 0;
+  double best = 0.0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  for (int w = 1; w <= 4 && e == hipSuccess && rc == W2V_OK; w *= 2) {
+    for (int bpc = 1; bpc <= c->occ_bpc[w] && e == hipSuccess && rc == W2V_OK; bpc++) {
+      const int blocks = bpc * c->num_cu;
+      if (max_in_flight > 0 && (int64_t)blocks * w > max_in_flight) continue;
+      e = hipMemcpyAsync(m0, k0, rows, hipMemcpyDeviceToDevice, s.st);
+      if (e == hipSuccess) e = hipMemcpyAsync(m1, k1, rows, hipMemcpyDeviceToDevice, s.st);
+      if (e != hipSuccess) break;
+      c->user_wpb = w;
+      c->user_blocks = blocks;
+      if ((e = hipEventRecord(e0, s.st)) != hipSuccess) break;
+      if ((rc = enqueue_device_step(c, s, epoch, sb, se, false))) break;
+      if ((e = hipEventRecord(e1, s.st)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess) break;
+      float ms = 0.0f;
+      if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+      const double wps = (double)n / (ms * 1e-3);
+      tried++;
+      if (wps > best) {
+        best = wps;
+        best_w = w;
+        best_b = blocks;
+      }
+    }
+  }
+  // restore whatever happened above
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  hipError_t e2 = hipMemcpyAsync(m0, k0, rows, hipMemcpyDeviceToDevice, s.st);
+  if (e2 == hipSuccess) e2 = hipMemcpyAsync(m1, k1, rows, hipMemcpyDeviceToDevice, s.st);
+  if (e2 == hipSuccess) e2 = hipMemcpyAsync(c->d_counters, cnt, sizeof(cnt), hipMemcpyHostToDevice, s.st);
+  if (e2 == hipSuccess) e2 = hipStreamSynchronize(s.st);
+  (void)hipFree(k0);
+  (void)hipFree(k1);
+  c->words_read = words0;
+  c->launches = launches0;
+  s.launches = slaunch0;
+  s.ms = sms0;
+  c->user_wpb = uw0;
+  c->user_blocks = ub0;
+  if (rc) return rc;
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(c, W2V_ERR_HIP, "launch-shape tuning failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  if (!tried) return fail(c, W2V_ERR_ARG, "no launch shape fits max_in_flight = %lld", (long long)max_in_flight);
+  c->user_wpb = best_w;
+  c->user_blocks = best_b;
+  out->waves_per_block = best_w;
+  out->blocks = best_b;
+  out->words_per_sec = best;
+  out->candidates = tried;
+  if (cache_path) {
+    if (FILE* f = fopen(cache_path, "a")) {
+      fprintf(f, "%s|%d %d %.6e\n", key, best_w, best_b, best);
+      fclose(f);
+    }
+  }
+  return W2V_OK;
 }
 
 int w2v_set_launch_shape(w2v_ctx* c, int32_t wpb, int32_t blocks) {

@@ -30,7 +30,7 @@ EXPORTS = [
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
     "w2v_synth_corpus", "w2v_synth_cluster_of", "w2v_corpus_info", "w2v_corpus_download",
     "w2v_corpus_counts", "w2v_graph_ba_edges", "w2v_graph_ba", "w2v_synth_walks",
-    "w2v_set_launch_shape", "w2v_get_launch_shape",
+    "w2v_autotune", "w2v_set_launch_shape", "w2v_get_launch_shape",
 ]
 
 
@@ -43,6 +43,12 @@ class Params(ctypes.Structure):
         ("device", ctypes.c_int32), ("streams", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
     ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("waves_per_block", ctypes.c_int32), ("blocks", ctypes.c_int32),
+                ("words_per_sec", ctypes.c_double), ("candidates", ctypes.c_int32),
+                ("cached", ctypes.c_int32)]
 
 
 class Walks(ctypes.Structure):
@@ -118,6 +124,7 @@ def lib() -> ctypes.CDLL:
             "w2v_graph_ba_edges": [i64, i32, ctypes.POINTER(i64)],
             "w2v_graph_ba": [i64, i32, u64, vp, vp],
             "w2v_synth_walks": [vp, vp, vp, i64, ctypes.POINTER(Walks)],
+            "w2v_autotune": [vp, i32, i64, i64, i64, ctypes.c_char_p, ctypes.POINTER(Tuning)],
             "w2v_set_launch_shape": [vp, i32, i32],
             "w2v_get_launch_shape": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)],
         }
@@ -344,6 +351,15 @@ class Trainer:
         c = np.zeros(self.V, np.uint64)
         self._chk(lib().w2v_corpus_counts(self.h, _ptr(c)))
         return c
+
+    def autotune(self, epoch, sent_begin, sent_end, max_in_flight=0, cache_path=None):
+        """Times the launch shapes on a sentence range (model restored), applies the
+        fastest; returns (waves_per_block, blocks, words_per_sec, candidates, cached)."""
+        t = Tuning()
+        cp = None if cache_path is None else os.fsencode(cache_path)
+        self._chk(lib().w2v_autotune(self.h, epoch, int(sent_begin), int(sent_end), int(max_in_flight), cp,
+                                     ctypes.byref(t)))
+        return t.waves_per_block, t.blocks, t.words_per_sec, t.candidates, bool(t.cached)
 
     def set_launch_shape(self, waves_per_block=0, blocks=0):
         self._chk(lib().w2v_set_launch_shape(self.h, int(waves_per_block), int(blocks)))

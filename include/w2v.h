@@ -189,6 +189,24 @@ int w2v_train_chunk(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int32_t* 
  * train concurrently. (0, 0) restores the default. No counterpart in the reference,
  * whose shape is fixed by -kernel-batch-size and d (kernels.cu:15-32). */
 int w2v_set_launch_shape(w2v_ctx* ctx, int32_t waves_per_block, int32_t blocks);
+/* Launch-shape tuning. Times every launch shape the context can run (1, 2 or 4 waves
+ * per workgroup, 1 up to the resident number of workgroups per CU, times the CU count),
+ * optionally capped at max_in_flight concurrent sentences (Hogwild staleness on small
+ * corpora), by training sentences [sent_begin, sent_end) of the device corpus in epoch
+ * `epoch` from a snapshot of the model; the model and the counters are restored after
+ * each trial, so tuning changes no training state. The fastest shape is applied as by
+ * w2v_set_launch_shape and returned. With cache_path, a line keyed by (device, kernel,
+ * storage, d, W, K, cap) is reused when present and appended otherwise. Hogwild runs
+ * only. No counterpart in the reference, whose shape is fixed by -kernel-batch-size. */
+typedef struct w2v_tuning {
+  int32_t waves_per_block;
+  int32_t blocks;
+  double words_per_sec; /* of the chosen shape on the tuning range */
+  int32_t candidates;   /* shapes timed (0 when taken from the cache) */
+  int32_t cached;
+} w2v_tuning;
+int w2v_autotune(w2v_ctx* ctx, int32_t epoch, int64_t sent_begin, int64_t sent_end, int64_t max_in_flight,
+                 const char* cache_path, w2v_tuning* out);
 /* Current launch shape (after defaults are resolved). */
 int w2v_get_launch_shape(w2v_ctx* ctx, int32_t* waves_per_block, int32_t* blocks);
 

@@ -1,4 +1,5 @@
-"""hipGraph-captured epochs (w2v_train_epoch_graph) against the eager device path.
+"""hipGraph-captured epochs (w2v_train_epoch_graph) against the eager device path, and
+launch-shape tuning (w2v_autotune).
 
 Deterministic mode (one wave, ordered flushes): every graph launch, including replays of
 the cached executable for a new epoch, leaves the model bit-identical to the same steps
@@ -93,4 +94,31 @@ def test_epoch_graph_rejects_bad_bounds(w2vmod):
             tr.train_epoch_graph(0, np.array(bad, np.int64))
     with pytest.raises(w2vmod.W2VError):
         tr.train_epoch_graph(5, np.array([0, ns], np.int64))  # epoch outside the schedule
+    tr.close()
+
+
+@pytest.mark.parametrize("kernel,storage", [("plain", "fp32"), ("mfma", "bf16")])
+def test_autotune_leaves_state_and_applies_fastest(w2vmod, tmp_path, kernel, storage):
+    tr, ns = make(w2vmod, kernel, storage, "hogwild", n_sent=30000, d=128)
+    before = tr.download()
+    st0 = tr.stats()
+    cache = str(tmp_path / "tune.txt")
+    w, b, wps, n, cached = tr.autotune(0, 0, ns // 2, cache_path=cache)
+    print(f"{kernel}: tuned {w} x {b} at {wps / 1e6:.1f} M words/s over {n} shapes", flush=True)
+    assert not cached and n >= 3 and wps > 0
+    assert (w, b) == tr.launch_shape()
+    after = tr.download()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert tr.stats() == st0
+    # capped concurrency, and the cache on the next call
+    w2, b2, _, n2, _ = tr.autotune(0, 0, ns // 2, max_in_flight=512)
+    assert w2 * b2 <= 512 and n2 >= 1
+    w3, b3, _, n3, cached3 = tr.autotune(0, 0, ns // 2, cache_path=cache)
+    assert cached3 and n3 == 0 and (w3, b3) == (w, b) == tr.launch_shape()
+    # deterministic contexts are not tuned
+    det, _ = make(w2vmod, kernel, storage, "deterministic", n_sent=100)
+    with pytest.raises(w2vmod.W2VError):
+        det.autotune(0, 0, 10)
+    det.close()
+    tr.train_device(0, 0, ns)  # trains with the tuned shape
     tr.close()
