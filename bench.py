@@ -89,6 +89,7 @@ def run_variant(w2v, kernel, storage, steps, warmup, host):
         host["subs"] = w2v.subsample_probs(cn, WL["sample"], nt)
         host["tokens"], host["nt"], host["ns"] = None, nt, ns
     tr.tables_upload(host["unigram"], host["subs"])
+    hot = tr.auto_hot_rows(host["counts"], WL["sample"], nt)
     epochs = max(1, math.ceil((steps + warmup) / CHUNKS))
     tr.set_schedule(WL["alpha"], epochs, nt)
     tr.init_model()
@@ -124,7 +125,7 @@ def run_variant(w2v, kernel, storage, steps, warmup, host):
     per_launch = kept / max(launches, 1) * bpw
     achieved = per_launch / (avg_ms / 1e3) / 1e9
     log(f"[{kernel}/{storage}] {words / secs / 1e6:.1f} M words/s, kept {kept / words:.3f}, "
-        f"kernel avg {avg_ms:.1f} ms, {achieved:.0f} GB/s algorithmic")
+        f"kernel avg {avg_ms:.1f} ms, {achieved:.0f} GB/s algorithmic, {hot} hot rows")
     # quality: held-out loss and planted-cluster recovery
     s0, _ = tr.download()
     hcfg = w2v.synth_config(WL["clusters"], WL["median"], WL["len_max"], WL["zipf"], WL["stay"],
@@ -143,7 +144,7 @@ def run_variant(w2v, kernel, storage, steps, warmup, host):
     return dict(kernel=kernel, storage=storage, words_per_sec=words / secs, seconds=secs,
                 words=int(words), kept_words=int(kept), launches=int(launches),
                 kernel_avg_ms=avg_ms, bytes_per_word=bpw, bytes_per_launch=per_launch,
-                achieved_gbs=achieved, heldout_loss=loss, heldout_pairs=int(pairs),
+                achieved_gbs=achieved, hot_rows=int(hot), heldout_loss=loss, heldout_pairs=int(pairs),
                 nn_recovery=rec, ms_per_step=secs / max(steps, 1) * 1e3)
 
 
@@ -161,6 +162,7 @@ def run_streamed(w2v, steps, warmup, host):
     nt, ns = tr.corpus_info()
     tok, off = tr.corpus_download(0, ns)  # the corpus in host memory
     tr.tables_upload(host["unigram"], host["subs"])
+    hot = tr.auto_hot_rows(host["counts"], WL["sample"], nt)
     epochs = max(1, math.ceil((steps + warmup) / CHUNKS))
     tr.set_schedule(WL["alpha"], epochs, nt)
     tr.init_model()
@@ -220,6 +222,7 @@ def run_walks(w2v, steps, warmup):
             cn = np.maximum(cn, 1)
             log(f"[walks/{kernel}/{mode}] corpus: {ns} walks, {T} tokens ({time.perf_counter() - t:.1f}s)")
             tr.tables_upload(w2v.unigram_table(cn, WL["unigram"]), w2v.subsample_probs(cn, c["sample"], T))
+            hot = tr.auto_hot_rows(cn, c["sample"], T)
             tr.set_schedule(c["alpha"], 2, T)
             tr.init_model()
             bounds = np.array([round(i * ns / steps) for i in range(steps + 1)], np.int64)
@@ -242,7 +245,7 @@ def run_walks(w2v, steps, warmup):
             st1 = tr.stats()
             words = st1["words_read"] - st0["words_read"]
             r = dict(words_per_sec=words / secs, seconds=secs, words=int(words),
-                     kept_words=int(st1["words_trained"] - st0["words_trained"]))
+                     kept_words=int(st1["words_trained"] - st0["words_trained"]), hot_rows=int(hot))
             if mode == "graph":
                 tr.synth_walks(off, adj, 1, c["walk_length"], c["trunc_min"], c["trunc_frac"],
                                seed=c["heldout_seed"])
@@ -252,6 +255,72 @@ def run_walks(w2v, steps, warmup):
             log(f"[walks/{kernel}/{mode}] {r['words_per_sec'] / 1e6:.1f} M words/s "
                 + (f"held-out loss {r['heldout_loss']:.4f}" if "heldout_loss" in r else ""))
             results[f"{kernel}_{mode}"] = r
+    return results
+
+
+COMBINED = dict(V=4_000_000, d=300, W=10, K=15, sample=1e-4, alpha=0.025, tokens=5_000_000_000,
+                clusters=256, zipf=1.05, median=22, len_max=4000, sigma=1.0, stay=0.9, in_cluster=0.8,
+                seed=1, heldout_seed=2, heldout_sent=5_000, parts=100)
+
+
+def run_combined(w2v, steps, warmup):
+    """The survey's combined configuration at full size: vocabulary 4M (Zipf 1.05, 256
+    clusters), 5B tokens in sentences of log-normal length (median 22, max 4000), d=300,
+    window 10, 15 negatives, model resident in HBM (bf16 rows for the MFMA kernel, FP32 for
+    the plain one). The epoch is split into 100 sentence ranges; `warmup` ranges are
+    trained untimed and the next `steps` timed (a bounded sample of the epoch)."""
+    c = COMBINED
+    cfg = w2v.synth_config(c["clusters"], c["median"], c["len_max"], c["zipf"], c["stay"], c["in_cluster"],
+                           c["sigma"], c["seed"])
+    mean_len = c["median"] * math.exp(c["sigma"] ** 2 / 2)
+    n_sent = int(c["tokens"] / mean_len)
+    results, host = {}, {}
+    for kernel, storage in variants(w2v):
+        tr = w2v.Trainer(c["V"], c["d"], c["W"], c["K"], msl=c["len_max"], storage=storage, kernel=kernel,
+                         sigmoid="table", mode="hogwild", seed=c["seed"])
+        t = time.perf_counter()
+        tr.synth_corpus(cfg, n_sent)
+        nt, ns = tr.corpus_info()
+        log(f"[combined/{kernel}] corpus: {ns} sentences, {nt} tokens ({time.perf_counter() - t:.1f}s)")
+        if not host:
+            cn = tr.corpus_counts()
+            cn[0] = ns
+            host["cn"] = cn
+            host["uni"] = w2v.unigram_table(cn, WL["unigram"])
+            host["subs"] = w2v.subsample_probs(cn, c["sample"], nt)
+        tr.tables_upload(host["uni"], host["subs"])
+        hot = tr.auto_hot_rows(host["cn"], c["sample"], nt)
+        tr.set_schedule(c["alpha"], 1, nt)
+        tr.init_model()
+        bounds = [round(i * ns / c["parts"]) for i in range(c["parts"] + 1)]
+        for i in range(warmup):
+            tr.train_device(0, bounds[i], bounds[i + 1])
+        tr.sync()
+        ms0, n0 = tr.kernel_time(0)
+        st0 = tr.stats()
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            tr.train_device(0, bounds[i], bounds[i + 1])
+            tr.sync()
+            log(f"[combined/{kernel}] step {i - warmup + 1}/{steps} {time.perf_counter() - t0:.1f}s")
+        secs = time.perf_counter() - t0
+        ms1, n1 = tr.kernel_time(0)
+        st1 = tr.stats()
+        words = st1["words_read"] - st0["words_read"]
+        kept = st1["words_trained"] - st0["words_trained"]
+        avg_ms = (ms1 - ms0) / max(n1 - n0, 1)
+        bpw = bytes_per_word(c["d"], c["K"], 2 if storage == "bf16" else 4)
+        hcfg = w2v.synth_config(c["clusters"], c["median"], c["len_max"], c["zipf"], c["stay"], c["in_cluster"],
+                                c["sigma"], c["heldout_seed"])
+        tr.synth_corpus(hcfg, c["heldout_sent"])
+        loss, _ = tr.heldout_loss(*tr.corpus_download(0, c["heldout_sent"]))
+        tr.close()
+        results[kernel] = dict(words_per_sec=words / secs, seconds=secs, words=int(words), kept_words=int(kept),
+                               kernel_avg_ms=avg_ms, achieved_gbs=kept / max(n1 - n0, 1) * bpw / (avg_ms / 1e3) / 1e9,
+                               heldout_loss_after_sample=loss, corpus_tokens=int(nt), corpus_sentences=int(ns),
+                               hot_rows=int(hot))
+        log(f"[combined/{kernel}] {words / secs / 1e6:.1f} M words/s, kernel avg {avg_ms:.0f} ms, "
+            f"{results[kernel]['achieved_gbs']:.0f} GB/s algorithmic, held-out loss {loss:.4f}")
     return results
 
 
@@ -325,7 +394,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--streamed", action="store_true",
                     help="also time the MFMA path with the corpus streamed from host memory")
-    ap.add_argument("--workload", default="text", help="text (the headline workload) | walks")
+    ap.add_argument("--workload", default="text", help="text (the headline workload) | walks | combined")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to scale HBM byte counters")
     args = ap.parse_args()
@@ -349,6 +418,20 @@ def main():
             "config": {"workload": "Barabasi-Albert graph (1M nodes, m=5), 10 walks x 80 per node, "
                                    "30% truncated to U[5,80], d=128 window 10 negative 5 sample 1e-4, 1 epoch",
                        "best": best[0], "parallelism": "single-gpu"},
+            "variants": res}), flush=True)
+        return
+    if args.workload == "combined":
+        res = run_combined(w2v, args.steps, args.warmup)
+        best = max(res.items(), key=lambda kv: kv[1]["words_per_sec"])
+        print(json.dumps({
+            "metric": "trained words/sec, 4M vocabulary, d=300 w=10 neg=15, sentences up to 4000 tokens",
+            "value": best[1]["words_per_sec"], "unit": "words/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16" if best[0] == "mfma" else "fp32", "data": "synthetic",
+            "config": {"workload": "cluster-Markov corpus, V=4M (Zipf 1.05, 256 clusters), 5B tokens, "
+                                   "lognormal sentences (median 22, max 4000), d=300 window 10 negative 15, "
+                                   f"{args.steps} of 100 epoch parts timed", "best": best[0],
+                       "parallelism": "single-gpu"},
             "variants": res}), flush=True)
         return
     host = {}

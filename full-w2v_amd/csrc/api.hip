@@ -92,6 +92,7 @@ struct w2v_ctx {
   int cfg_blocks[2] = {0, 0};
   int max_blocks[2] = {0, 0};  // device-filling grid at cfg_wpb
   int user_wpb = 0, user_blocks = 0;  // w2v_set_launch_shape override
+  int32_t hot_rows = W2V_DEFAULT_HOT_ROWS;  // w2v_set_hot_rows
   int occ_bpc[5] = {0, 0, 0, 0, 0};   // Hogwild: resident workgroups per CU at 1..4 waves each
   std::string arch;                   // device name (gcnArchName), the tuning cache key
   float* dsyn0 = nullptr;
@@ -224,6 +225,7 @@ UpdateArgs base_args(w2v_ctx* c) {
   a.msl = c->p.max_sentence_length;
   a.sig_exact = c->p.sigmoid == W2V_SIGMOID_EXACT;
   a.fence = c->p.mode == W2V_MODE_DETERMINISTIC;
+  a.hot_rows = c->hot_rows;
   a.exp_t = c->d_exp;
   a.nexp_t = c->d_nexp;
   a.counters = c->d_counters;
@@ -1112,6 +1114,46 @@ int w2v_set_launch_shape(w2v_ctx* c, int32_t wpb, int32_t blocks) {
   if (!c || wpb < 0 || wpb > 4 || blocks < 0) return W2V_ERR_ARG;
   c->user_wpb = wpb;
   c->user_blocks = blocks;
+  return W2V_OK;
+}
+
+int w2v_set_hot_rows(w2v_ctx* c, int64_t n) {
+  if (!c) return W2V_ERR_ARG;
+  if (n < 0) return fail(c, W2V_ERR_ARG, "hot row count must be >= 0 (got %lld)", (long long)n);
+  c->hot_rows = (int32_t)(n < c->p.vocab_size ? n : c->p.vocab_size);
+  return W2V_OK;
+}
+
+int w2v_auto_hot_rows(w2v_ctx* c, const uint64_t* cn, float sample, uint64_t train_words, int64_t* out) {
+  if (!c || !cn) return W2V_ERR_ARG;
+  const int64_t V = c->p.vocab_size;
+  int32_t wpb = 0, blocks = 0;
+  int rc = w2v_get_launch_shape(c, &wpb, &blocks);
+  if (rc) return rc;
+  // expected copies of word i held at once = sentences in flight x ring slots x the
+  // word's share of the kept tokens; a word is hot when copies x (context, target)
+  // pairs per centre reaches the threshold
+  const double in_flight = (double)wpb * blocks, slots = 2.0 * c->p.window + 1;
+  const double pairs = 2.0 * c->p.window * (c->p.negative + 1);
+  const double base = (double)sample * (double)train_words;
+  std::vector<double> kept((size_t)V);
+  double total = 0.0;
+  for (int64_t i = 0; i < V; i++) {
+    double k = (double)cn[i];
+    if (sample > 0 && cn[i] > 0) {  // keep probability (improved_driver_functions.cpp:71-77)
+      const double r = base / (double)cn[i];
+      const double p = sqrt(r) + r;
+      if (p < 1.0) k *= p;
+    }
+    kept[(size_t)i] = k;
+    total += k;
+  }
+  int64_t h = 0;
+  if (total > 0)
+    for (int64_t i = 0; i < V; i++)
+      if (in_flight * slots * (kept[(size_t)i] / total) * pairs >= W2V_HOT_THRESHOLD) h = i + 1;
+  c->hot_rows = (int32_t)h;
+  if (out) *out = h;
   return W2V_OK;
 }
 

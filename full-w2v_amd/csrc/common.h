@@ -131,6 +131,9 @@ struct UpdateArgs {
   int32_t d, P, W, K;
   int32_t sig_exact;
   int32_t fence;  // deterministic ordering of flushes before later reads
+  // Hogwild instantiations: context rows with token id < hot_rows are written back and
+  // reloaded after every centre instead of once when they leave the window (0: never)
+  int32_t hot_rows;
   const float* exp_t;
   const float* nexp_t;
   // work queue and counters

@@ -42,6 +42,7 @@ struct Options {
   // options of this implementation
   std::string precision = "fp32", save_ckpt, resume;
   int device_sampling = 0, device = 0, exact_sigmoid = 0;
+  long long hot_rows = -1;  // -1: from the vocabulary counts (w2v_auto_hot_rows)
   unsigned long long seed = 1;
 };
 
@@ -82,6 +83,8 @@ void usage(const char* prog) {
   printf("\t-precision <fp32|bf16>\n\t\tfp32: per-pair FP32 kernel; bf16: bf16 rows, MFMA window tile\n");
   printf("\t-device-sampling <0|1>\n\t\tSubsample and draw negatives on the GPU from a resident corpus\n");
   printf("\t-exact-sigmoid <0|1>\n\t\tUse 1/(1+exp(-x)) instead of the 1000-entry table\n");
+  printf("\t-hot-rows <int>\n\t\tContext rows of the n most frequent words are written back and reloaded\n"
+         "\t\tafter every centre (-1: chosen from the word counts; 0: off)\n");
   printf("\t-device <int>\n\t-seed <int>\n");
   printf("\t-save-checkpoint <file>\n\t\tWrite model, syn1neg and counters to <file> at every epoch end\n");
   printf("\t-resume <file>\n\t\tContinue from a checkpoint (same vocabulary, sizes and -threads)\n");
@@ -150,6 +153,7 @@ Options parse(int argc, char** argv) {
   num("-exact-sigmoid", o.exact_sigmoid);
   num("-device", o.device);
   if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = strtoull(argv[i + 1], nullptr, 10);
+  if ((i = arg_pos("-hot-rows", argc, argv)) > 0 && i + 1 < argc) o.hot_rows = strtoll(argv[i + 1], nullptr, 10);
   if (o.debug > 0) {
     fprintf(stderr, "** FILES & I/O **\n");
     if (!o.do_export && !o.output.empty()) fprintf(stderr, "!!\tProcess will train but will not export a model!\n");
@@ -474,6 +478,13 @@ int main(int argc, char** argv) {
   if (o.sample > 0) {
     subs.resize(V);
     w2v_subsample_probs(cn.data(), V, o.sample, vocab.train_words, subs.data());
+  }
+  if (o.hot_rows < 0) {
+    int64_t h = 0;
+    if ((rc = w2v_auto_hot_rows(ctx, cn.data(), o.sample, vocab.train_words, &h))) die("w2v_auto_hot_rows", ctx, rc);
+    if (o.debug > 0) fprintf(stderr, "Hot rows: %lld\n", (long long)h);
+  } else if ((rc = w2v_set_hot_rows(ctx, o.hot_rows))) {
+    die("w2v_set_hot_rows", ctx, rc);
   }
   if (o.pre_shuffle) corpus.shuffle();
   if (o.debug > 0) fprintf(stderr, "Network ready\n");

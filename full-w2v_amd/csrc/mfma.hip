@@ -14,7 +14,11 @@ MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw) {
   l.off_u0 = o;
   if (!u0_regs) o = al(o + R * P * (delta ? 4 : 2));
   l.off_ts = o;
-  o = al(o + NT * l.PT * 2);
+  // the target stage also receives the hot-row reloads in Hogwild runs (HOT_STAGE rows
+  // of NP 256-byte chunks), after the S tile has consumed it
+  const int np = (P + 127) / 128;
+  const int stage = NT * l.PT * 2, hot = hw ? HOT_STAGE * np * 256 : 0;
+  o = al(o + (stage > hot ? stage : hot));
   l.off_gs = o;
   o = al(o + (R > 16 ? 32 : 16) * 16 * 4);
   l.wave_bytes = o;

@@ -26,6 +26,8 @@ namespace w2v {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void* lds_vptr;
+typedef __attribute__((address_space(1))) void* glb_vptr;
 
 struct MfmaLayout {
   int PU;  // FP32 ring row pitch (floats), padded against LDS bank conflicts
@@ -36,11 +38,26 @@ struct MfmaLayout {
 // Hogwild runs with one 16-row tile and rows of at most 384 elements keep the load-time
 // copy of the context ring (bf16 pairs) in registers instead of LDS, which lets more
 // waves fit in a CU's LDS.
+// Hot rows refreshed per centre through the target stage (Hogwild runs).
+constexpr int HOT_STAGE = 4;
+// Cache policy of the hot-row reloads: SC1 (agent scope) reads L2, where every wave's
+// atomics land, instead of a vector-L1 line that predates this wave's own flushes.
+constexpr int CPOL_SC1 = 16;
+
 __host__ __device__ constexpr bool u0_in_regs(bool hw, int mt, int np) {
   return hw && mt == 1 && np <= 3;
 }
 
 MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw);
+
+// Ring reads and writes between the hot-row global->LDS loads and their wait. They are
+// plain float accesses (type-based alias information): the compiler then does not
+// make them wait for those loads, which only ever write the target stage.
+__device__ __forceinline__ float2 lds_ld2(const float* p) { return make_float2(p[0], p[1]); }
+__device__ __forceinline__ void lds_st2(float* p, float x, float y) {
+  p[0] = x;
+  p[1] = y;
+}
 
 __device__ __forceinline__ float bf_lo(uint32_t p) { return __uint_as_float(p << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
@@ -383,6 +400,49 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
         }
       }
 
+      // ---- hot rows (Hogwild): the context rows of frequent words that stay in the
+      // window are written back and reloaded after each centre, so a working copy is at
+      // most one centre behind the other waves' updates instead of 2W+1 (with thousands
+      // of sentences in flight, a head word's stale copies otherwise add up their deltas
+      // and the row diverges). Up to HOT_STAGE such rows per centre, taken in a rotating
+      // order, are reloaded straight into the target stage (free once S is computed) by
+      // global->LDS loads that complete while G, dC and dT are computed; they are
+      // swapped in before this centre's atomics.
+      int hsl[HOT_STAGE], htk[HOT_STAGE];
+#pragma unroll
+      for (int q = 0; q < HOT_STAGE; q++) hsl[q] = -1, htk[q] = 0;
+      bool hot_any = false;
+      if (pipe && a.hot_rows > 0) {
+        int rel = lane - cs;
+        if (rel < 0) rel += R;
+        if (rel > W) rel -= R;
+        const bool hot = lane < R && rel != -W && w + rel >= 0 && w + rel < L && ring_tok < a.hot_rows;
+        const uint64_t hm = __ballot(hot);
+        if (hm) {
+          hot_any = true;
+          const int rot = w % R;
+          uint64_t rm = ((hm >> rot) | (hm << (R - rot))) & ((1ull << R) - 1);
+#pragma unroll
+          for (int q = 0; q < HOT_STAGE; q++) {
+            if (rm) {
+              int c = __builtin_ctzll(rm) + rot;
+              rm &= rm - 1;
+              if (c >= R) c -= R;
+              hsl[q] = c;
+              htk[q] = readlane_i(ring_tok, c);
+#pragma unroll
+              for (int e = 0; e < NP; e++) {
+                const int j = 2 * lane + 128 * e;
+                if (in_p(e, j))
+                  __builtin_amdgcn_global_load_lds((glb_vptr)(a.bsyn0 + (size_t)htk[q] * P + j),
+                                                   (lds_vptr)((unsigned char*)Ts + (q * NP + e) * 256), 4, 0,
+                                                   CPOL_SC1);
+              }
+            }
+          }
+        }
+      }
+
       // ---- G = g(S) on the valid (context, target) cells; lane holds S[16m+4*g4+i][r16]
 #pragma unroll
       for (int m = 0; m < MT; m++) {
@@ -422,7 +482,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
         for (int e = 0; e < NP; e++) {
           const int j = 2 * lane + 128 * e;
           if (in_p(e, j)) {
-            const float2 u = *(const float2*)(uf + j);
+            const float2 u = lds_ld2(uf + j);
             float du0 = 0.0f, du1 = 0.0f;
 #pragma unroll
             for (int k = 0; k < NTB; k++) {
@@ -431,7 +491,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
               dT[k][e][0] = fmaf(gk[k], u.x, dT[k][e][0]);
               dT[k][e][1] = fmaf(gk[k], u.y, dT[k][e][1]);
             }
-            *(float2*)(uf + j) = make_float2(u.x + du0, u.y + du1);
+            lds_st2(uf + j, u.x + du0, u.y + du1);
           }
         }
       }
@@ -476,6 +536,29 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
         }
       }
 
+      if (hot_any) {  // swap the reloaded hot rows in: flush each delta, then rebase
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the global->LDS reloads have landed
+#pragma unroll
+        for (int q = 0; q < HOT_STAGE; q++) {
+          if (hsl[q] < 0) continue;
+          float* uf = Uf + hsl[q] * ly.PU;
+          uint32_t op[NP], fr[NP];
+          u0_get(hsl[q], op);
+#pragma unroll
+          for (int e = 0; e < NP; e++) {
+            const int j = 2 * lane + 128 * e;
+            fr[e] = in_p(e, j) ? ((const uint32_t*)((const unsigned char*)Ts + (q * NP + e) * 256))[lane] : 0u;
+            if (in_d(e, j)) {
+              const float2 u = *(const float2*)(uf + j);
+              sr_add_pair(a.bsyn0 + (size_t)htk[q] * P + j, u.x - bf_lo(op[e]), u.y - bf_hi(op[e]), op[e],
+                          in_d(e, j + 1), rng);
+            }
+            if (in_p(e, j)) *(float2*)(uf + j) = make_float2(bf_lo(fr[e]), bf_hi(fr[e]));
+          }
+          u0_put(hsl[q], fr);
+        }
+      }
+
       // ---- target deltas back to the model
 #pragma unroll
       for (int k = 0; k < NTB; k++) {
@@ -517,6 +600,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
           load_row(slot, enter_tok);
         }
       }
+
       __builtin_amdgcn_wave_barrier();
       cs = (cs + 1 == R) ? 0 : cs + 1;
     }

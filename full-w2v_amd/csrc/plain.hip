@@ -167,6 +167,11 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
         dst[e] = v;
       }
     };
+    auto fetch_fresh = [&](const float* m, int row, float* dst) {
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        dst[e] = __hip_atomic_load(m + (size_t)row * P + e * 64 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
     auto entering_token = [&](int pos) -> int {  // pos advances by one per centre
       if (pos - chunk0 >= 64) {
         chunk0 += 64;
@@ -355,6 +360,58 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
           for (int e = 0; e < E; e++) {
             u0[e * 64 + lane] = nE[e];
             dd[e * 64 + lane] = 0.0f;
+          }
+        }
+        // hot rows: a frequent word's context row that stays in the window is written
+        // back and reloaded after every centre (at most one centre stale instead of
+        // 2W+1; many stale copies of a head word otherwise sum their deltas and diverge).
+        // The reloads are agent-scope loads (L2, where the atomics land) issued before
+        // this centre's flushes.
+        if (a.hot_rows > 0) {
+          int rel = lane - cs;
+          if (rel < 0) rel += R;
+          if (rel > W) rel -= R;
+          const bool hot = lane < R && rel != -W && w + rel >= 0 && w + rel < L && ring_tok < a.hot_rows;
+          uint64_t hm = __ballot(hot);
+          while (hm) {  // up to HB rows per batch: their reloads are in flight together
+            constexpr int HB = E <= 4 ? 4 : 2;
+            int hs[HB], ht[HB];
+#pragma unroll
+            for (int q = 0; q < HB; q++) {
+              hs[q] = -1;
+              ht[q] = 0;
+              if (hm) {
+                hs[q] = __builtin_ctzll(hm);
+                hm &= hm - 1;
+                ht[q] = readlane_i(ring_tok, hs[q]);
+              }
+            }
+            float fr[HB][E];
+#pragma unroll
+            for (int q = 0; q < HB; q++)
+              if (hs[q] >= 0) fetch_fresh(a.syn0, ht[q], fr[q]);
+#pragma unroll
+            for (int q = 0; q < HB; q++) {
+              if (hs[q] < 0) continue;
+              const float* dd = Ds + (size_t)hs[q] * P;
+              float* dst = a.out0 + (size_t)ht[q] * P;
+#pragma unroll
+              for (int e = 0; e < E; e++) {
+                const int j = e * 64 + lane;
+                if (e < E - 1 || j < d) atomicAdd(dst + j, dd[j]);
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < HB; q++) {
+              if (hs[q] < 0) continue;
+              float* u0 = U0s + (size_t)hs[q] * P;
+              float* dd = Ds + (size_t)hs[q] * P;
+#pragma unroll
+              for (int e = 0; e < E; e++) {
+                u0[e * 64 + lane] = fr[q][e];
+                dd[e * 64 + lane] = 0.0f;
+              }
+            }
           }
         }
         if (leave >= 0) {

@@ -30,7 +30,7 @@ EXPORTS = [
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
     "w2v_synth_corpus", "w2v_synth_cluster_of", "w2v_corpus_info", "w2v_corpus_download",
     "w2v_corpus_counts", "w2v_graph_ba_edges", "w2v_graph_ba", "w2v_synth_walks",
-    "w2v_autotune", "w2v_set_launch_shape", "w2v_get_launch_shape",
+    "w2v_autotune", "w2v_set_launch_shape", "w2v_get_launch_shape", "w2v_set_hot_rows", "w2v_auto_hot_rows",
 ]
 
 
@@ -126,6 +126,8 @@ def lib() -> ctypes.CDLL:
             "w2v_synth_walks": [vp, vp, vp, i64, ctypes.POINTER(Walks)],
             "w2v_autotune": [vp, i32, i64, i64, i64, ctypes.c_char_p, ctypes.POINTER(Tuning)],
             "w2v_set_launch_shape": [vp, i32, i32],
+            "w2v_set_hot_rows": [vp, i64],
+            "w2v_auto_hot_rows": [vp, vp, ctypes.c_float, u64, ctypes.POINTER(i64)],
             "w2v_get_launch_shape": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)],
         }
         for name, args in sig.items():
@@ -363,6 +365,20 @@ class Trainer:
 
     def set_launch_shape(self, waves_per_block=0, blocks=0):
         self._chk(lib().w2v_set_launch_shape(self.h, int(waves_per_block), int(blocks)))
+
+    def set_hot_rows(self, n):
+        """Hogwild: context rows of token ids < n are written back and reloaded after
+        every centre (0 disables the refresh)."""
+        self._chk(lib().w2v_set_hot_rows(self.h, int(n)))
+
+    def auto_hot_rows(self, counts, sample, train_words):
+        """Sets the hot-row count from word counts and the launch shape; returns it."""
+        c = np.ascontiguousarray(counts, np.uint64)
+        if len(c) != self.V:
+            raise ValueError("counts must have one entry per vocabulary word")
+        out = ctypes.c_int64()
+        self._chk(lib().w2v_auto_hot_rows(self.h, _ptr(c), float(sample), int(train_words), ctypes.byref(out)))
+        return out.value
 
     def launch_shape(self):
         w, b = ctypes.c_int32(), ctypes.c_int32()

@@ -189,6 +189,32 @@ int w2v_train_chunk(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int32_t* 
  * train concurrently. (0, 0) restores the default. No counterpart in the reference,
  * whose shape is fixed by -kernel-batch-size and d (kernels.cu:15-32). */
 int w2v_set_launch_shape(w2v_ctx* ctx, int32_t waves_per_block, int32_t blocks);
+/* Hot rows of the Hogwild update kernels: a context row whose token id is below `n` is
+ * written back and reloaded after every centre while it is in the window, instead of
+ * once when it leaves (the lifetime reuse of kernels.cu:117-120,217-249). With
+ * thousands of sentences in flight, a head word sits in hundreds of windows at once;
+ * each copy holds 2W+1 centres of updates computed from a stale row and the summed
+ * deltas overshoot (observed at window 10 with 15 negatives: held-out loss diverging
+ * to 50-120 with the hold, decreasing steadily with 1024 hot rows). Token ids are taken
+ * to be sorted by decreasing frequency, as the reference's vocabulary is (SortVocab),
+ * so the first n ids are the n most frequent words. 0 disables; values above the
+ * vocabulary are clamped. Deterministic and delta runs hold every row for its lifetime.
+ * Default W2V_DEFAULT_HOT_ROWS. No counterpart in the reference. */
+#define W2V_DEFAULT_HOT_ROWS 1024
+int w2v_set_hot_rows(w2v_ctx* ctx, int64_t n);
+/* Hot-row count from the word counts (as given to w2v_subsample_probs; sample <= 0 means
+ * no subsampling) and the current launch shape: word i is hot when
+ *   (sentences in flight) x (2W+1) x (its share of the tokens kept by subsampling)
+ *   x 2W(1+K)  >=  W2V_HOT_THRESHOLD,
+ * i.e. when the stale copies of its row held at once, weighted by the updates each copy
+ * takes per centre, reach the threshold. Sets 1 + the largest such id (so an unsorted
+ * vocabulary is covered too) as by w2v_set_hot_rows and returns it in *out. Call after
+ * any w2v_set_launch_shape. The threshold separates the measured cases: 60 pairs per
+ * centre at 2048 sentences in flight needs about 256 hot words, 320 pairs at 768 about
+ * 1024. No counterpart in the reference. */
+#define W2V_HOT_THRESHOLD 512.0
+int w2v_auto_hot_rows(w2v_ctx* ctx, const uint64_t* counts, float sample, uint64_t train_words,
+                      int64_t* out);
 /* Launch-shape tuning. Times every launch shape the context can run (1, 2 or 4 waves
  * per workgroup, 1 up to the resident number of workgroups per CU, times the CU count),
  * optionally capped at max_in_flight concurrent sentences (Hogwild staleness on small

@@ -157,6 +157,12 @@ static double bf16_round(double v) {
 void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
                            const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
                            int sig_mode, int tile, int bf16_emul) {
+  orc_sgns_sentence_hot_f64(syn0, syn1, d, W, K, sen, L, neg, alpha, sig_mode, tile, bf16_emul, 0, 0);
+}
+
+void orc_sgns_sentence_hot_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
+                               const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
+                               int sig_mode, int tile, int bf16_emul, int32_t hot, int32_t hot_cap) {
   pthread_once(&g_tables_once, init_tables);
   if (L <= 0) return;
   /* per-position working copies U and load-time snapshots U0 (lifetime reuse) */
@@ -243,6 +249,28 @@ void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int
         if (c == w) continue;
         for (int32_t j = 0; j < d; j++) U[(size_t)c * d + j] += dU[(size_t)nc * d + j];
         nc++;
+      }
+    }
+    /* hot rows: positions w-W+1..w+W holding a word below `hot` (ring slot p mod R),
+     * taken in slot order from slot w mod R, at most hot_cap of them (0: all): the
+     * working row's delta goes to the model and the row restarts from the model value
+     * read before that write (so it lacks this centre's own update) */
+    if (hot > 0) {
+      const int32_t R = 2 * W + 1, rot = w % R;
+      int32_t taken = 0;
+      for (int32_t k = 0; k < R && (hot_cap <= 0 || taken < hot_cap); k++) {
+        const int32_t slot = (rot + k) % R;
+        int32_t p = w - W + ((slot - (w - W) % R) % R + R) % R;  /* p in [w-W, w+W], p mod R = slot */
+        if (p == w - W || p < 0 || p >= L || sen[p] >= hot) continue;
+        taken++;
+        double* u = U + (size_t)p * d;
+        double* u0 = U0 + (size_t)p * d;
+        double* m = syn0 + (size_t)sen[p] * d;
+        for (int32_t j = 0; j < d; j++) {
+          const double fresh = m[j];
+          m[j] += u[j] - u0[j];
+          u[j] = u0[j] = fresh;
+        }
       }
     }
     /* ring advance (kernels.cu:217-238): flush the leaving context, then load the

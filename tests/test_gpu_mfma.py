@@ -28,14 +28,14 @@ def run_mfma(w2v, V, d, W, K, syn0, syn1, tokens, off, neg, alpha, sigmoid="exac
     return b0, b1, g0, g1
 
 
-def oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile, bf16):
+def oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile, bf16, hot=0, hot_cap=0):
     s0, s1 = b0.astype(np.float64), b1.astype(np.float64)
     a0, a1 = s0.copy(), s1.copy()
     for s in range(len(off) - 1):
         t = tokens[off[s]:off[s + 1]]
         n = neg[off[s] * K:off[s + 1] * K] if K else None
         a0, a1 = orc.sgns_sentence(a0, a1, W, K, t, n, float(np.float32(alpha[s])), sig_mode=1,
-                                   tile=tile, bf16=bf16)
+                                   tile=tile, bf16=bf16, hot=hot, hot_cap=hot_cap)
     return a0 - s0, a1 - s1
 
 
@@ -125,10 +125,13 @@ def test_device_init_matches_initnet(orc, w2vmod, storage, kernel):
     assert np.array_equal(s0, ref) and not s1.any()
 
 
-def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod):
+@pytest.mark.parametrize("hot", ["none", "all"])
+def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot):
     """Hogwild mode with bf16 rows and stochastically rounded atomic write-back, one wave,
     no revisited rows: the model change must follow the tile oracle up to rounding noise
-    (correlation and unbiased scale), which catches wrong rows, signs or pairing."""
+    (correlation and unbiased scale), which catches wrong rows, signs or pairing.
+    hot="all": the hot-row refresh on every context row (up to 4 per centre, reloaded
+    through the target stage), against the oracle with the same refresh."""
     from test_gpu_plain import distinct_batch
 
     V, d, W, K = 4000, 64, 5, 5
@@ -139,16 +142,28 @@ def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod):
     alpha = np.full(len(lens), 0.25, np.float32)
     tr = w2vmod.Trainer(V, d, W, K, msl=64, storage="bf16", kernel="mfma", sigmoid="exact", seed=3)
     tr.set_launch_shape(1, 1)
+    tr.set_hot_rows(V if hot == "all" else 0)
     tr.upload(syn0, syn1)
     b0, b1 = tr.download()
     tr.submit_csr(tokens, off, neg, alpha)
     a0, a1 = tr.download()
     tr.close()
-    t0, t1 = oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile=1, bf16=1)
+    t0, t1 = oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile=1, bf16=1,
+                    hot=V if hot == "all" else 0, hot_cap=4)
     for got, ref in ((a0 - b0, t0), (a1 - b1, t1)):
         m = np.abs(ref) > 0
         x, y = got[m].astype(np.float64), ref[m]
         corr = np.corrcoef(x, y)[0, 1]
         scale = (x @ y) / (y @ y)
-        print(f"bf16 Hogwild path: corr {corr:.4f} scale {scale:.4f}", flush=True)
+        print(f"bf16 Hogwild path (hot {hot}): corr {corr:.4f} scale {scale:.4f}", flush=True)
         assert corr > 0.95 and abs(scale - 1) < 0.05
+    # the refresh changes the result well beyond rounding noise: the GPU change must sit
+    # much closer to the oracle with its own semantics than to the other one
+    o0, o1 = oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile=1, bf16=1,
+                    hot=0 if hot == "all" else V, hot_cap=4)
+    got = np.concatenate([(a0 - b0).ravel(), (a1 - b1).ravel()]).astype(np.float64)
+    own = np.linalg.norm(got - np.concatenate([t0.ravel(), t1.ravel()]))
+    other = np.linalg.norm(got - np.concatenate([o0.ravel(), o1.ravel()]))
+    print(f"bf16 Hogwild path (hot {hot}): distance to own oracle {own:.3e}, to the other {other:.3e}",
+          flush=True)
+    assert own < 0.5 * other

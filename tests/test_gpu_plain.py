@@ -53,14 +53,15 @@ def gpu_delta(w2v, V, d, W, K, syn0, syn1, tokens, off, neg, alpha, sigmoid="exa
     return d0, d1
 
 
-def oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha, sig_mode=1):
+def oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha, sig_mode=1, hot=0, hot_cap=0, tile=0):
     s0 = syn0.astype(np.float64)
     s1 = syn1.astype(np.float64)
     a0, a1 = s0.copy(), s1.copy()
     for s in range(len(off) - 1):
         t = tokens[off[s]:off[s + 1]]
         n = neg[off[s] * K:off[s + 1] * K] if K else None
-        a0, a1 = orc.sgns_sentence(a0, a1, W, K, t, n, float(np.float32(alpha[s])), sig_mode=sig_mode)
+        a0, a1 = orc.sgns_sentence(a0, a1, W, K, t, n, float(np.float32(alpha[s])), sig_mode=sig_mode,
+                                   tile=tile, hot=hot, hot_cap=hot_cap)
     return a0 - s0, a1 - s1
 
 
@@ -304,15 +305,19 @@ def test_hogwild_path_one_wave_matches_oracle(orc, w2vmod):
     assert e0 <= TOL and e1 <= TOL
 
 
-@pytest.mark.parametrize("sigmoid", ["exact", "table"])
-def test_hogwild_fast_path_one_wave_matches_oracle(orc, w2vmod, sigmoid):
+@pytest.mark.parametrize("sigmoid,hot", [("exact", "none"), ("table", "none"), ("exact", "all")])
+def test_hogwild_fast_path_one_wave_matches_oracle(orc, w2vmod, sigmoid, hot):
     """The Hogwild instantiation (no delta buffers: target groups on the skewed schedule,
     register-evaluated sigmoid buckets, rows prefetched a group ahead) on one wave, rows
     never revisited, updating the FP32 model in place: the model change must equal the
     serialised FP64 update. The in-place FP32 add rounds row + delta once per element; that
     half ulp of the stored value is taken off each element's error before the row-relative
     check. Table-sigmoid trials may straddle a bucket edge (FP32 vs FP64 dot product) now
-    and then."""
+    and then. hot="all": every context row is written back and reloaded after each
+    centre (the hot-row refresh), against the oracle with the same refresh; exact sigmoid
+    only, since each reload puts FP32-rounded rows back into the window and so makes
+    table-bucket straddles several times likelier (the table path is the same code as
+    in the hold trials)."""
     def stored_rel_err(after, before, ref):
         got = after.astype(np.float64) - before.astype(np.float64)
         half_ulp = np.spacing(np.abs(after)).astype(np.float64) / 2
@@ -333,13 +338,15 @@ def test_hogwild_fast_path_one_wave_matches_oracle(orc, w2vmod, sigmoid):
         alpha = np.full(len(lens), 0.25, np.float32)
         tr = w2vmod.Trainer(V, d, W, K, msl=64, sigmoid=sigmoid, mode="hogwild")
         tr.set_launch_shape(1, 1)
+        tr.set_hot_rows(V if hot == "all" else 0)
         tr.upload(syn0, syn1)
         tr.submit_csr(tokens, off, neg, alpha)
         a0, a1 = tr.download()
         tr.close()
         r0, r1 = oracle_delta(orc, syn0, syn1, W, K, tokens, off, neg, alpha,
-                              sig_mode=1 if sigmoid == "exact" else 0)
+                              sig_mode=1 if sigmoid == "exact" else 0, hot=V if hot == "all" else 0)
         e0, e1 = stored_rel_err(a0, syn0, r0), stored_rel_err(a1, syn1, r1)
-        print(f"hogwild fast path ({sigmoid}), 1 wave: syn0 rel {e0:.2e} syn1 rel {e1:.2e}", flush=True)
+        print(f"hogwild fast path ({sigmoid}, hot {hot}), 1 wave: syn0 rel {e0:.2e} syn1 rel {e1:.2e}",
+              flush=True)
         passed += e0 <= TOL and e1 <= TOL
     assert passed >= trials - (1 if sigmoid == "table" else 0)

@@ -183,6 +183,64 @@ def test_one_pair_each_centre_closed_form(orc, tile):
     assert np.allclose(r1, e1, rtol=0, atol=1e-15)
 
 
+def test_hot_refresh_closed_form(orc):
+    """Hot-row refresh, hand-traced on L=3, W=2, K=0, sentence (1, 2, 3), sequential order.
+    After each centre every staying position writes its delta to syn0 and restarts from
+    the syn0 value read before that write, so a row updated at one centre enters the
+    next centre without that update: centre 1 sees C (not C1), centre 2 sees A (not A1)
+    and B1. With at most one refresh per centre (slot order from w mod 5) the refreshed
+    positions are 0, then 1, then 2. hot=0 is the lifetime hold."""
+    rng = np.random.default_rng(11)
+    d, a = 8, 0.2
+    s0, s1 = rand_model(5, d, rng, ts=1.0)
+    s0 = s0 * 40  # rows large enough that every update matters
+    sen = np.array([1, 2, 3], np.int32)
+    u = s0.astype(np.float64)
+    v = s1.astype(np.float64)
+    A, B, C = u[1], u[2], u[3]
+    g = lambda x: a * (1 - sigmoid(x))
+
+    def train(t, ctx):  # one target against contexts in order; returns t', updated contexts
+        out = []
+        for x in ctx:
+            gg = g(x @ t)
+            x, t = x + gg * t, t + gg * x
+            out.append(x)
+        return t, out
+
+    # all rows hot, no cap
+    v1, (B1, C1) = train(v[1], [B, C])
+    v2, (A1, C2) = train(v[2], [A, C])
+    v3, (A2, B2) = train(v[3], [A, B1])
+    e0, e1 = u.copy(), v.copy()
+    e0[1], e0[2], e0[3] = A1 + A2 - A, B2, C1 + C2 - C
+    e1[1], e1[2], e1[3] = v1, v2, v3
+    r0, r1 = orc.sgns_sentence(s0, s1, 2, 0, sen, None, a, sig_mode=1, tile=0, hot=5)
+    assert np.allclose(r0, e0, rtol=0, atol=1e-13) and np.allclose(r1, e1, rtol=0, atol=1e-13)
+
+    # one refresh per centre: position 0 (no-op), then 1, then 2
+    w1, (B1, C1) = train(v[1], [B, C])
+    w2, (A1, C2) = train(v[2], [A, C1])
+    w3, (A2, B2) = train(v[3], [A1, B])
+    e0, e1 = u.copy(), v.copy()
+    e0[1], e0[2], e0[3] = A2, B1 + B2 - B, C2
+    e1[1], e1[2], e1[3] = w1, w2, w3
+    r0, r1 = orc.sgns_sentence(s0, s1, 2, 0, sen, None, a, sig_mode=1, tile=0, hot=5, hot_cap=1)
+    assert np.allclose(r0, e0, rtol=0, atol=1e-13) and np.allclose(r1, e1, rtol=0, atol=1e-13)
+
+    # the hold: every update stays in the working rows
+    h1, (B1, C1) = train(v[1], [B, C])
+    h2, (A1, C2) = train(v[2], [A, C1])
+    h3, (A2, B2) = train(v[3], [A1, B1])
+    e0, e1 = u.copy(), v.copy()
+    e0[1], e0[2], e0[3] = A2, B2, C2
+    e1[1], e1[2], e1[3] = h1, h2, h3
+    r0, r1 = orc.sgns_sentence(s0, s1, 2, 0, sen, None, a, sig_mode=1, tile=0)
+    assert np.allclose(r0, e0, rtol=0, atol=1e-13) and np.allclose(r1, e1, rtol=0, atol=1e-13)
+    q0, q1 = orc.sgns_sentence(s0, s1, 2, 0, sen, None, a, sig_mode=1, tile=0, hot=1)  # no word < 1
+    assert np.array_equal(q0, r0) and np.array_equal(q1, r1)
+
+
 def first_order_delta(s0, s1, W, K, sen, neg, a):
     """Sum of per-pair SGNS gradients at the initial point (alpha -> 0 limit)."""
     u = s0.astype(np.float64)
