@@ -401,13 +401,15 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       }
 
       // ---- hot rows (Hogwild): the context rows of frequent words that stay in the
-      // window are written back and reloaded after each centre, so a working copy is at
-      // most one centre behind the other waves' updates instead of 2W+1 (with thousands
-      // of sentences in flight, a head word's stale copies otherwise add up their deltas
-      // and the row diverges). Up to HOT_STAGE such rows per centre, taken in a rotating
-      // order, are reloaded straight into the target stage (free once S is computed) by
-      // global->LDS loads that complete while G, dC and dT are computed; they are
-      // swapped in before this centre's atomics.
+      // window are reloaded after each centre, so a working copy is at most one centre
+      // behind the other waves' updates instead of 2W+1 (with thousands of sentences in
+      // flight, a head word's stale copies otherwise add up their deltas and the row
+      // diverges). Their deltas are published every second centre (odd centres; on even
+      // ones the row is rebased onto the reloaded value with its delta kept), which
+      // halves the atomics on the most contended rows. Up to HOT_STAGE such rows per
+      // centre, taken in a rotating order, are reloaded straight into the target stage
+      // (free once S is computed) by global->LDS loads that complete while G, dC and dT
+      // are computed; they are swapped in before this centre's atomics.
       int hsl[HOT_STAGE], htk[HOT_STAGE];
 #pragma unroll
       for (int q = 0; q < HOT_STAGE; q++) hsl[q] = -1, htk[q] = 0;
@@ -548,12 +550,18 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
           for (int e = 0; e < NP; e++) {
             const int j = 2 * lane + 128 * e;
             fr[e] = in_p(e, j) ? ((const uint32_t*)((const unsigned char*)Ts + (q * NP + e) * 256))[lane] : 0u;
-            if (in_d(e, j)) {
+            if (w & 1) {  // odd centre: publish the delta, restart from the reloaded row
+              if (in_d(e, j)) {
+                const float2 u = *(const float2*)(uf + j);
+                sr_add_pair(a.bsyn0 + (size_t)htk[q] * P + j, u.x - bf_lo(op[e]), u.y - bf_hi(op[e]), op[e],
+                            in_d(e, j + 1), rng);
+              }
+              if (in_p(e, j)) *(float2*)(uf + j) = make_float2(bf_lo(fr[e]), bf_hi(fr[e]));
+            } else if (in_p(e, j)) {  // even centre: rebase onto the reloaded row, delta kept
               const float2 u = *(const float2*)(uf + j);
-              sr_add_pair(a.bsyn0 + (size_t)htk[q] * P + j, u.x - bf_lo(op[e]), u.y - bf_hi(op[e]), op[e],
-                          in_d(e, j + 1), rng);
+              *(float2*)(uf + j) =
+                  make_float2(bf_lo(fr[e]) + (u.x - bf_lo(op[e])), bf_hi(fr[e]) + (u.y - bf_hi(op[e])));
             }
-            if (in_p(e, j)) *(float2*)(uf + j) = make_float2(bf_lo(fr[e]), bf_hi(fr[e]));
           }
           u0_put(hsl[q], fr);
         }

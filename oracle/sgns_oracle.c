@@ -157,12 +157,13 @@ static double bf16_round(double v) {
 void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
                            const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
                            int sig_mode, int tile, int bf16_emul) {
-  orc_sgns_sentence_hot_f64(syn0, syn1, d, W, K, sen, L, neg, alpha, sig_mode, tile, bf16_emul, 0, 0);
+  orc_sgns_sentence_hot_f64(syn0, syn1, d, W, K, sen, L, neg, alpha, sig_mode, tile, bf16_emul, 0, 0, 1);
 }
 
 void orc_sgns_sentence_hot_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
                                const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
-                               int sig_mode, int tile, int bf16_emul, int32_t hot, int32_t hot_cap) {
+                               int sig_mode, int tile, int bf16_emul, int32_t hot, int32_t hot_cap,
+                               int32_t flush_every) {
   pthread_once(&g_tables_once, init_tables);
   if (L <= 0) return;
   /* per-position working copies U and load-time snapshots U0 (lifetime reuse) */
@@ -252,9 +253,11 @@ void orc_sgns_sentence_hot_f64(double* syn0, double* syn1, int32_t d, int32_t W,
       }
     }
     /* hot rows: positions w-W+1..w+W holding a word below `hot` (ring slot p mod R),
-     * taken in slot order from slot w mod R, at most hot_cap of them (0: all): the
-     * working row's delta goes to the model and the row restarts from the model value
-     * read before that write (so it lacks this centre's own update) */
+     * taken in slot order from slot w mod R, at most hot_cap of them (0: all). On a
+     * publishing centre (every one, or the odd ones with flush_every = 2) the working
+     * row's delta goes to the model and the row restarts from the model value read before
+     * that write (so it lacks this centre's own update); otherwise the row is rebased onto
+     * the model value with its delta kept */
     if (hot > 0) {
       const int32_t R = 2 * W + 1, rot = w % R;
       int32_t taken = 0;
@@ -266,10 +269,18 @@ void orc_sgns_sentence_hot_f64(double* syn0, double* syn1, int32_t d, int32_t W,
         double* u = U + (size_t)p * d;
         double* u0 = U0 + (size_t)p * d;
         double* m = syn0 + (size_t)sen[p] * d;
-        for (int32_t j = 0; j < d; j++) {
-          const double fresh = m[j];
-          m[j] += u[j] - u0[j];
-          u[j] = u0[j] = fresh;
+        if (flush_every <= 1 || w % flush_every == flush_every - 1) {
+          for (int32_t j = 0; j < d; j++) {
+            const double fresh = m[j];
+            m[j] += u[j] - u0[j];
+            u[j] = u0[j] = fresh;
+          }
+        } else {
+          for (int32_t j = 0; j < d; j++) {
+            const double fresh = m[j];
+            u[j] = fresh + (u[j] - u0[j]);
+            u0[j] = fresh;
+          }
         }
       }
     }

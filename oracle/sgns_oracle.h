@@ -62,15 +62,18 @@ void orc_sgns_sentence_f64(double* syn0, double* syn1, int32_t d, int32_t W, int
                            int sig_mode, int tile, int bf16_emul);
 /* The same with the Hogwild kernels' hot-row refresh (no counterpart in the reference):
  * after each centre w, every context position p in w-W+1..w+W whose word is below `hot`
- * writes its working-row delta to syn0 and restarts from the syn0 value read before that
- * write. Positions are visited in ring-slot order (slot p mod (2W+1)) starting at slot
- * w mod (2W+1), at most hot_cap per centre (0 = all), as the MFMA kernel stages them;
- * the FP32 kernel refreshes all (hot_cap 0). The refresh runs before the leaving
- * position's flush. Pinned by: hot = 0 reproduces orc_sgns_sentence_f64 bit for bit;
- * a refresh with nothing in between is a no-op on the final model (tests). */
+ * is refreshed from syn0. On a publishing centre it writes its working-row delta to syn0
+ * and restarts from the syn0 value read before that write; otherwise (flush_every = 2,
+ * even centres) the working row becomes that syn0 value plus its unpublished delta.
+ * flush_every = 1 publishes at every centre (the FP32 kernel), 2 at odd centres (the
+ * MFMA kernel). Positions are visited in ring-slot order (slot p mod (2W+1)) starting at
+ * slot w mod (2W+1), at most hot_cap per centre (0 = all; the MFMA kernel stages 4).
+ * The refresh runs before the leaving position's flush. Pinned by hand-traced closed
+ * forms on a three-word sentence (tests/test_oracle.py). */
 void orc_sgns_sentence_hot_f64(double* syn0, double* syn1, int32_t d, int32_t W, int32_t K,
                                const int32_t* sen, int32_t L, const int32_t* neg, double alpha,
-                               int sig_mode, int tile, int bf16_emul, int32_t hot, int32_t hot_cap);
+                               int sig_mode, int tile, int bf16_emul, int32_t hot, int32_t hot_cap,
+                               int32_t flush_every);
 
 /* ---- sentence assembly: FULL-W2V-driver.cu:102-114,166-302 ----
  * Restates one host thread's batching loop over a sentence slice for one epoch:

@@ -59,7 +59,7 @@ def lib():
         L.orc_sgns_sentence_f64.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, f64, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int]
         L.orc_sgns_sentence_hot_f64.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, f64, ctypes.c_int,
-                                                ctypes.c_int, ctypes.c_int, i32, i32]
+                                                ctypes.c_int, ctypes.c_int, i32, i32, i32]
         L.orc_assemble_epoch.argtypes = [vp, vp, i64, i64, i32, i32, f32, vp, vp, i64, f32, i32, u64,
                                          ctypes.POINTER(ThreadState), vp, vp, vp, vp, i64, i64]
         L.orc_assemble_epoch.restype = i64
@@ -128,17 +128,18 @@ def init_syn0(V, d):
     return s
 
 
-def sgns_sentence(syn0, syn1, W, K, sen, neg, alpha, sig_mode=1, tile=0, bf16=0, hot=0, hot_cap=0):
+def sgns_sentence(syn0, syn1, W, K, sen, neg, alpha, sig_mode=1, tile=0, bf16=0, hot=0, hot_cap=0,
+                  flush_every=1):
     """FP64 update of one sentence in place on float64 copies; returns (syn0', syn1').
     hot > 0 adds the Hogwild kernels' hot-row refresh (words < hot, at most hot_cap per
-    centre, 0 = all)."""
+    centre, 0 = all; deltas published every flush_every-th centre)."""
     s0 = np.array(syn0, np.float64, order="C")
     s1 = np.array(syn1, np.float64, order="C")
     sen = np.ascontiguousarray(sen, np.int32)
     neg = np.ascontiguousarray(neg if neg is not None else np.zeros(0), np.int32)
     if hot:
         lib().orc_sgns_sentence_hot_f64(P(s0), P(s1), s0.shape[1], W, K, P(sen), len(sen), P(neg), alpha,
-                                        sig_mode, tile, bf16, hot, hot_cap)
+                                        sig_mode, tile, bf16, hot, hot_cap, flush_every)
     else:
         lib().orc_sgns_sentence_f64(P(s0), P(s1), s0.shape[1], W, K, P(sen), len(sen), P(neg), alpha,
                                     sig_mode, tile, bf16)

@@ -28,14 +28,14 @@ def run_mfma(w2v, V, d, W, K, syn0, syn1, tokens, off, neg, alpha, sigmoid="exac
     return b0, b1, g0, g1
 
 
-def oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile, bf16, hot=0, hot_cap=0):
+def oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile, bf16, hot=0, hot_cap=0, flush_every=1):
     s0, s1 = b0.astype(np.float64), b1.astype(np.float64)
     a0, a1 = s0.copy(), s1.copy()
     for s in range(len(off) - 1):
         t = tokens[off[s]:off[s + 1]]
         n = neg[off[s] * K:off[s + 1] * K] if K else None
         a0, a1 = orc.sgns_sentence(a0, a1, W, K, t, n, float(np.float32(alpha[s])), sig_mode=1,
-                                   tile=tile, bf16=bf16, hot=hot, hot_cap=hot_cap)
+                                   tile=tile, bf16=bf16, hot=hot, hot_cap=hot_cap, flush_every=flush_every)
     return a0 - s0, a1 - s1
 
 
@@ -131,7 +131,8 @@ def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot):
     no revisited rows: the model change must follow the tile oracle up to rounding noise
     (correlation and unbiased scale), which catches wrong rows, signs or pairing.
     hot="all": the hot-row refresh on every context row (up to 4 per centre, reloaded
-    through the target stage), against the oracle with the same refresh."""
+    through the target stage, deltas published at odd centres), against the oracle with
+    the same refresh."""
     from test_gpu_plain import distinct_batch
 
     V, d, W, K = 4000, 64, 5, 5
@@ -149,7 +150,7 @@ def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot):
     a0, a1 = tr.download()
     tr.close()
     t0, t1 = oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile=1, bf16=1,
-                    hot=V if hot == "all" else 0, hot_cap=4)
+                    hot=V if hot == "all" else 0, hot_cap=4, flush_every=2)
     for got, ref in ((a0 - b0, t0), (a1 - b1, t1)):
         m = np.abs(ref) > 0
         x, y = got[m].astype(np.float64), ref[m]
@@ -160,7 +161,7 @@ def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot):
     # the refresh changes the result well beyond rounding noise: the GPU change must sit
     # much closer to the oracle with its own semantics than to the other one
     o0, o1 = oracle(orc, b0, b1, W, K, tokens, off, neg, alpha, tile=1, bf16=1,
-                    hot=0 if hot == "all" else V, hot_cap=4)
+                    hot=0 if hot == "all" else V, hot_cap=4, flush_every=2)
     got = np.concatenate([(a0 - b0).ravel(), (a1 - b1).ravel()]).astype(np.float64)
     own = np.linalg.norm(got - np.concatenate([t0.ravel(), t1.ravel()]))
     other = np.linalg.norm(got - np.concatenate([o0.ravel(), o1.ravel()]))

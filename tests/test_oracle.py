@@ -189,7 +189,8 @@ def test_hot_refresh_closed_form(orc):
     the syn0 value read before that write, so a row updated at one centre enters the
     next centre without that update: centre 1 sees C (not C1), centre 2 sees A (not A1)
     and B1. With at most one refresh per centre (slot order from w mod 5) the refreshed
-    positions are 0, then 1, then 2. hot=0 is the lifetime hold."""
+    positions are 0, then 1, then 2. Publishing every second centre: centre 0 only
+    rebases, centre 1 publishes, so centre 2 sees A and B. hot=0 is the lifetime hold."""
     rng = np.random.default_rng(11)
     d, a = 8, 0.2
     s0, s1 = rand_model(5, d, rng, ts=1.0)
@@ -226,6 +227,18 @@ def test_hot_refresh_closed_form(orc):
     e0[1], e0[2], e0[3] = A2, B1 + B2 - B, C2
     e1[1], e1[2], e1[3] = w1, w2, w3
     r0, r1 = orc.sgns_sentence(s0, s1, 2, 0, sen, None, a, sig_mode=1, tile=0, hot=5, hot_cap=1)
+    assert np.allclose(r0, e0, rtol=0, atol=1e-13) and np.allclose(r1, e1, rtol=0, atol=1e-13)
+
+    # publishing at odd centres only: centre 0 rebases (updates kept, as in the hold),
+    # centre 1 publishes (working rows restart from the model value before the write),
+    # centre 2 rebases onto the model value with its own delta kept
+    x1, (B1, C1) = train(v[1], [B, C])
+    x2, (A1, C2) = train(v[2], [A, C1])
+    x3, (A2, B2) = train(v[3], [A, B])
+    e0, e1 = u.copy(), v.copy()
+    e0[1], e0[2], e0[3] = A1 + A2 - A, B1 + B2 - B, C2
+    e1[1], e1[2], e1[3] = x1, x2, x3
+    r0, r1 = orc.sgns_sentence(s0, s1, 2, 0, sen, None, a, sig_mode=1, tile=0, hot=5, flush_every=2)
     assert np.allclose(r0, e0, rtol=0, atol=1e-13) and np.allclose(r1, e1, rtol=0, atol=1e-13)
 
     # the hold: every update stays in the working rows
