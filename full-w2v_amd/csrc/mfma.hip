@@ -3,10 +3,19 @@
 
 namespace w2v {
 
+static int nt_bucket(int nt) {
+  if (nt <= 2) return 2;
+  if (nt <= 4) return 4;
+  if (nt <= 6) return 6;
+  if (nt <= 8) return 8;
+  if (nt <= 12) return 12;
+  return 16;
+}
+
 MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw) {
   MfmaLayout l;
   const int R = 2 * W + 1, NT = K + 1;
-  const bool u0_regs = u0_in_regs(hw, R > 16 ? 2 : 1, (P + 127) / 128);
+  const bool u0_regs = u0_in_regs(hw, R > 16 ? 2 : 1, (P + 127) / 128, nt_bucket(NT));
   l.PU = P + 4;
   l.PT = P + 8;
   auto al = [](int x) { return (x + 15) / 16 * 16; };
@@ -25,14 +34,6 @@ MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw) {
   return l;
 }
 
-static int nt_bucket(int nt) {
-  if (nt <= 2) return 2;
-  if (nt <= 4) return 4;
-  if (nt <= 6) return 6;
-  if (nt <= 8) return 8;
-  if (nt <= 12) return 12;
-  return 16;
-}
 
 static const void* mfma_select(int P, int NT, int R, bool hw) {
   const int np = (P + 127) / 128, ntb = nt_bucket(NT), mt = R > 16 ? 2 : 1;

@@ -35,17 +35,20 @@ struct MfmaLayout {
   int off_u0, off_ts, off_gs, wave_bytes;
 };
 
-// Hogwild runs with one 16-row tile and rows of at most 384 elements keep the load-time
-// copy of the context ring (bf16 pairs) in registers instead of LDS, which lets more
-// waves fit in a CU's LDS.
 // Hot rows refreshed per centre through the target stage (Hogwild runs).
 constexpr int HOT_STAGE = 4;
 // Cache policy of the hot-row reloads: SC1 (agent scope) reads L2, where every wave's
 // atomics land, instead of a vector-L1 line that predates this wave's own flushes.
 constexpr int CPOL_SC1 = 16;
 
-__host__ __device__ constexpr bool u0_in_regs(bool hw, int mt, int np) {
-  return hw && mt == 1 && np <= 3;
+// Hogwild runs with rows of at most 384 elements keep the load-time copy of the context
+// ring (bf16 pairs) in registers instead of LDS, which lets more waves fit in a CU's LDS:
+// always with one 16-row tile (16 slots), and with two tiles (up to 24 slots) in the
+// shapes that run one wave per SIMD anyway (NP x NTB > 18), whose 512 registers hold it.
+// (At d=300, window 10, 15 negatives this takes LDS from 52 KB to 39 KB per wave: 4 waves
+// per CU instead of 3.)
+__host__ __device__ constexpr bool u0_in_regs(bool hw, int mt, int np, int ntb) {
+  return hw && np <= 3 && (mt == 1 || np * ntb > 18);
 }
 
 MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw);
@@ -126,7 +129,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
   const bool fence = HW ? false : a.fence != 0;
   auto in_p = [&](int e, int j) { return e < NP - 1 || j < P; };
   auto in_d = [&](int e, int j) { return e < NP - 1 || j < d; };
-  constexpr bool U0R = u0_in_regs(HW, MT, NP);
+  constexpr bool U0R = u0_in_regs(HW, MT, NP, NTB);
   unsigned char* wb = lds_raw + (size_t)wib * ly.wave_bytes;
   float* Uf = (float*)wb;
   unsigned char* U0 = wb + ly.off_u0;
@@ -135,7 +138,9 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
   const int cpb = K > 0 ? 64 / K : 0;
   unsigned long long words = 0, sents = 0;
   typedef uint32_t u32x16_t __attribute__((ext_vector_type(16)));
+  typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
   u32x16_t u0v[NP];  // load-time ring pairs when held in registers (element = ring slot)
+  u32x8_t u0w[MT == 2 ? NP : 1];  // slots 16..23 of a two-tile ring
 
   for (;;) {
     unsigned s32 = 0;
@@ -195,8 +200,13 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
     // compiles to VGPR-indexed moves (no private memory, no select chains).
     auto u0_put = [&](int slot, const uint32_t* v) {
       if constexpr (U0R) {
+        if (MT == 1 || slot < 16) {
 #pragma unroll
-        for (int e = 0; e < NP; e++) u0v[e][slot] = v[e];
+          for (int e = 0; e < NP; e++) u0v[e][slot] = v[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < NP; e++) u0w[MT == 2 ? e : 0][slot - 16] = v[e];
+        }
       } else {
 #pragma unroll
         for (int e = 0; e < NP; e++) {
@@ -207,8 +217,13 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
     };
     auto u0_get = [&](int slot, uint32_t* v) {
       if constexpr (U0R) {
+        if (MT == 1 || slot < 16) {
 #pragma unroll
-        for (int e = 0; e < NP; e++) v[e] = u0v[e][slot];
+          for (int e = 0; e < NP; e++) v[e] = u0v[e][slot];
+        } else {
+#pragma unroll
+          for (int e = 0; e < NP; e++) v[e] = u0w[MT == 2 ? e : 0][slot - 16];
+        }
       } else {
 #pragma unroll
         for (int e = 0; e < NP; e++) {

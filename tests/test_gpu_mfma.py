@@ -125,17 +125,19 @@ def test_device_init_matches_initnet(orc, w2vmod, storage, kernel):
     assert np.array_equal(s0, ref) and not s1.any()
 
 
+@pytest.mark.parametrize("shape", [(4000, 64, 5, 5), (4000, 300, 10, 15)])
 @pytest.mark.parametrize("hot", ["none", "all"])
-def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot):
+def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot, shape):
     """Hogwild mode with bf16 rows and stochastically rounded atomic write-back, one wave,
     no revisited rows: the model change must follow the tile oracle up to rounding noise
     (correlation and unbiased scale), which catches wrong rows, signs or pairing.
     hot="all": the hot-row refresh on every context row (up to 4 per centre, reloaded
     through the target stage, deltas published at odd centres), against the oracle with
-    the same refresh."""
+    the same refresh. Shapes: one 16-row tile, and the window-10 / 15-negative shape with
+    two tiles and the load-time ring copy held in registers."""
     from test_gpu_plain import distinct_batch
 
-    V, d, W, K = 4000, 64, 5, 5
+    V, d, W, K = shape
     rng = np.random.default_rng(41)
     syn0, syn1 = model(V, d, rng, t_scale=1.0)
     lens = [30, 30, 30, 30]
@@ -156,7 +158,7 @@ def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot):
         x, y = got[m].astype(np.float64), ref[m]
         corr = np.corrcoef(x, y)[0, 1]
         scale = (x @ y) / (y @ y)
-        print(f"bf16 Hogwild path (hot {hot}): corr {corr:.4f} scale {scale:.4f}", flush=True)
+        print(f"bf16 Hogwild path {shape} (hot {hot}): corr {corr:.4f} scale {scale:.4f}", flush=True)
         assert corr > 0.95 and abs(scale - 1) < 0.05
     # the refresh changes the result well beyond rounding noise: the GPU change must sit
     # much closer to the oracle with its own semantics than to the other one
@@ -165,6 +167,6 @@ def test_mfma_hogwild_bf16_path_tracks_oracle(orc, w2vmod, hot):
     got = np.concatenate([(a0 - b0).ravel(), (a1 - b1).ravel()]).astype(np.float64)
     own = np.linalg.norm(got - np.concatenate([t0.ravel(), t1.ravel()]))
     other = np.linalg.norm(got - np.concatenate([o0.ravel(), o1.ravel()]))
-    print(f"bf16 Hogwild path (hot {hot}): distance to own oracle {own:.3e}, to the other {other:.3e}",
-          flush=True)
+    print(f"bf16 Hogwild path {shape} (hot {hot}): distance to own oracle {own:.3e}, to the other "
+          f"{other:.3e}", flush=True)
     assert own < 0.5 * other
