@@ -43,12 +43,13 @@ constexpr int CPOL_SC1 = 16;
 
 // Hogwild runs with rows of at most 384 elements keep the load-time copy of the context
 // ring (bf16 pairs) in registers instead of LDS, which lets more waves fit in a CU's LDS:
-// always with one 16-row tile (16 slots), and with two tiles (up to 24 slots) in the
-// shapes that run one wave per SIMD anyway (NP x NTB > 18), whose 512 registers hold it.
+// one 16-slot vector per row pair for one 16-row tile, plus an 8-slot one for two tiles.
+// The two-tile copy (NP x 24 registers) fits every shape's register budget except
+// NP x NTB = 18 (two waves per SIMD, already at 254 of 256 VGPRs), which keeps it in LDS.
 // (At d=300, window 10, 15 negatives this takes LDS from 52 KB to 39 KB per wave: 4 waves
-// per CU instead of 3.)
+// per CU instead of 3; at d=128, window 10, 5 negatives from 20 KB to 15 KB.)
 __host__ __device__ constexpr bool u0_in_regs(bool hw, int mt, int np, int ntb) {
-  return hw && np <= 3 && (mt == 1 || np * ntb > 18);
+  return hw && np <= 3 && (mt == 1 || np * ntb != 18);
 }
 
 MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw);
