@@ -402,8 +402,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the trainer has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
     import w2v
 
