@@ -190,7 +190,9 @@ int w2v_train_chunk(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int32_t* 
  * whose shape is fixed by -kernel-batch-size and d (kernels.cu:15-32). */
 int w2v_set_launch_shape(w2v_ctx* ctx, int32_t waves_per_block, int32_t blocks);
 /* Hot rows of the Hogwild update kernels: a context row whose token id is below `n` is
- * written back and reloaded after every centre while it is in the window, instead of
+ * reloaded after every centre while it is in the window, and its delta written back at
+ * every centre (FP32 kernel) or every second centre (MFMA kernel, which keeps the
+ * unpublished delta on top of the reloaded row), instead of being read once and written
  * once when it leaves (the lifetime reuse of kernels.cu:117-120,217-249). With
  * thousands of sentences in flight, a head word sits in hundreds of windows at once;
  * each copy holds 2W+1 centres of updates computed from a stale row and the summed
