@@ -5,8 +5,8 @@
 //  * host sampling (default, the reference's Train_SGNS, FULL-W2V-driver.cu:80-354):
 //    -threads host threads each own a contiguous slice of sentences, draw subsampling
 //    and negatives from their LCG stream (seed = thread id), build GPU sentences of up
-//    to -msl kept tokens and submit batches of -kernel-batch-size sentences round-robin
-//    over -streams library streams;
+//    to -msl kept tokens and submit batches of -kernel-batch-size sentences (at most
+//    2^20 tokens) round-robin over -streams library streams;
 //  * device sampling (-device-sampling 1): the corpus is streamed to the GPU in chunks of
 //    whole sentences (pinned staging, two alternating streams) and each chunk is trained
 //    with subsampling, negatives and the learning rate computed on the device.
@@ -31,13 +31,14 @@ using namespace w2vhost;
 namespace {
 
 constexpr int64_t kUnigramTableSize = 100000000;  // UNIGRAM_TABLE_SIZE (FULL-W2V-driver.cu:61)
+constexpr int64_t kBatchTokens = (int64_t)1 << 20;  // token cap of one host batch
 
 struct Options {
   std::string output, save_vocab, read_vocab, train, save_corpus, save_cache, read_cache;
   int binary = 0, debug = 2;
   bool recount = false, do_export = true, cache_eos = false, pre_shuffle = false, epoch_shuffle = false;
   int negative = 5, min_count = 5, threads = 12, streams = 4, epochs = 5, window = 5,
-      kernel_batch = 200, layer = 100, msl = 1000;
+      kernel_batch = 20000, layer = 100, msl = 1000;
   float alpha = 0.025f, sample = 1e-3f;
   // options of this implementation
   std::string precision = "fp32", save_ckpt, resume;
@@ -76,7 +77,8 @@ void usage(const char* prog) {
   printf("\t-negative(s) <int>\n\t\tNumber of negative examples; default is 5\n");
   printf("** PARALLEL UTILIZATION **\n");
   printf("\t-thread(s) <int>\n\t\tUse <int> host threads (default 12)\n");
-  printf("\t-kernel-batch-size <int>\n\t\tSentences per kernel launch; default is 200\n");
+  printf("\t-kernel-batch-size <int>\n\t\tSentences per kernel launch (at most 2^20 tokens); default is 20000\n"
+         "\t\t(the reference's 200 leaves most of an MI355X idle: one wave trains one sentence)\n");
   printf("\t-streams <int>\n\t\tGPU streams each host thread manages; default is 4\n");
   printf("\t-max-sentence-length, -msl <int>\n\t\tThe maximum length of a single sentence\n");
   printf("** MI355X OPTIONS **\n");
@@ -349,7 +351,9 @@ void* train_thread(void* p) {
     }
     boff.push_back((int64_t)btok.size());
     balpha.push_back(sh.alpha);
-    if ((int)balpha.size() == o.kernel_batch) submit();
+    // a batch also ends at kBatchTokens tokens, which bounds each stream's pinned staging
+    // when sentences are long (-msl chunks without -cache-eos)
+    if ((int)balpha.size() == o.kernel_batch || (int64_t)btok.size() >= kBatchTokens) submit();
   }
   submit();
   pthread_barrier_wait(&sh.bar);
