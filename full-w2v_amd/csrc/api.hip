@@ -64,6 +64,10 @@ struct StreamState {
   unsigned int* queue = nullptr;
   // staging for host batches: one batch in flight per stream
   HostBuf h_tok, h_neg, h_len, h_alpha, h_off;
+  // host batches queued for one launch (w2v_set_coalesce): CSR tokens, negatives, alpha
+  std::vector<int32_t> q_tok, q_neg;
+  std::vector<int64_t> q_off = std::vector<int64_t>(1, 0);
+  std::vector<float> q_alpha;
   DevBuf d_tok, d_neg, d_len, d_alpha, d_off;
   // device-corpus step buffers
   DevBuf flags, pos, kept, kept_off, temp;
@@ -93,6 +97,7 @@ struct w2v_ctx {
   int max_blocks[2] = {0, 0};  // device-filling grid at cfg_wpb
   int user_wpb = 0, user_blocks = 0;  // w2v_set_launch_shape override
   int32_t hot_rows = W2V_DEFAULT_HOT_ROWS;  // w2v_set_hot_rows
+  int64_t coalesce_sent = W2V_DEFAULT_COALESCE_SENTENCES, coalesce_tok = W2V_DEFAULT_COALESCE_TOKENS;
   int occ_bpc[5] = {0, 0, 0, 0, 0};   // Hogwild: resident workgroups per CU at 1..4 waves each
   std::string arch;                   // device name (gcnArchName), the tuning cache key
   float* dsyn0 = nullptr;
@@ -391,9 +396,44 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
   return W2V_OK;
 }
 
+static int launch_csr(w2v_ctx* c, StreamState& s, const int32_t* tokens, const int64_t* offsets,
+                      int64_t n_sent, const int32_t* neg, const float* alpha);
+
+// Host batches are queued per stream and launched together (Hogwild runs without delta
+// buffers): one sentence per wave, so a launch of a few hundred sentences leaves most of
+// the device idle. Deterministic and delta runs launch every batch as it comes.
+static bool coalescing(const w2v_ctx* c) {
+  return c->coalesce_sent > 0 && c->p.mode == W2V_MODE_HOGWILD && !c->delta_mode;
+}
+static bool queue_full(const w2v_ctx* c, const StreamState& s) {
+  return (int64_t)s.q_alpha.size() >= c->coalesce_sent || (int64_t)s.q_tok.size() >= c->coalesce_tok;
+}
+static int flush_queue(w2v_ctx* c, StreamState& s) {
+  if (s.q_alpha.empty()) return W2V_OK;
+  const int rc = launch_csr(c, s, s.q_tok.data(), s.q_off.data(), (int64_t)s.q_alpha.size(),
+                            c->p.negative > 0 ? s.q_neg.data() : nullptr, s.q_alpha.data());
+  s.q_tok.clear();
+  s.q_neg.clear();
+  s.q_off.assign(1, 0);
+  s.q_alpha.clear();
+  return rc;
+}
+
+int w2v_set_coalesce(w2v_ctx* c, int64_t max_sentences, int64_t max_tokens) {
+  if (!c) return W2V_ERR_ARG;
+  if (max_sentences < 0 || max_tokens < 0 || max_sentences >= ((int64_t)1 << 31))
+    return fail(c, W2V_ERR_ARG, "coalescing limits must be >= 0 and below 2^31 sentences");
+  int rc = w2v_sync_all(c);  // launch what is queued under the old limits
+  if (rc) return rc;
+  c->coalesce_sent = max_sentences;
+  c->coalesce_tok = max_tokens > 0 ? max_tokens : INT64_MAX;
+  return W2V_OK;
+}
+
 int w2v_destroy(w2v_ctx* c) {
   if (!c) return W2V_OK;
   if (c->p.device >= 0) hipSetDevice(c->p.device);
+  for (auto& s : c->ss) (void)flush_queue(c, s);  // queued batches are still trained
   for (auto& s : c->ss) {
     if (s.st) hipStreamSynchronize(s.st);
     for (auto& pr : s.pending) {
@@ -431,6 +471,10 @@ int w2v_sync_all(w2v_ctx* c) {
   if (!c) return W2V_ERR_ARG;
   HIP_TRY(c, hipSetDevice(c->p.device));
   for (auto& s : c->ss) {
+    int rq = flush_queue(c, s);
+    if (rq) return rq;
+  }
+  for (auto& s : c->ss) {
     HIP_TRY(c, hipStreamSynchronize(s.st));
     int rc = drain_timing(c, s);
     if (rc) return rc;
@@ -442,6 +486,7 @@ int w2v_sync(w2v_ctx* c, int32_t stream) {
   int rc = check_stream(c, stream);
   if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->p.device));
+  if ((rc = flush_queue(c, c->ss[stream]))) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->ss[stream].st));
   return drain_timing(c, c->ss[stream]);
 }
@@ -668,6 +713,23 @@ int w2v_submit_csr(w2v_ctx* c, int32_t stream, const int32_t* tokens, const int6
   if ((rc = validate_ids(c, tokens, nt, "tokens"))) return rc;
   if (K > 0 && (rc = validate_ids(c, neg, nt * K, "neg"))) return rc;
   StreamState& s = c->ss[stream];
+  if (coalescing(c)) {
+    const int64_t base = (int64_t)s.q_tok.size();
+    s.q_tok.insert(s.q_tok.end(), tokens, tokens + nt);
+    if (K > 0) s.q_neg.insert(s.q_neg.end(), neg, neg + nt * K);
+    for (int64_t i = 1; i <= n_sent; i++) s.q_off.push_back(base + offsets[i]);
+    s.q_alpha.insert(s.q_alpha.end(), alpha, alpha + n_sent);
+    return queue_full(c, s) ? flush_queue(c, s) : W2V_OK;
+  }
+  return launch_csr(c, s, tokens, offsets, n_sent, neg, alpha);
+}
+
+// Copies one CSR batch into the stream's pinned staging, uploads it and launches it.
+static int launch_csr(w2v_ctx* c, StreamState& s, const int32_t* tokens, const int64_t* offsets,
+                      int64_t n_sent, const int32_t* neg, const float* alpha) {
+  int rc;
+  const int K = c->p.negative;
+  const int64_t nt = offsets[n_sent];
   HIP_TRY(c, hipSetDevice(c->p.device));
   HIP_TRY(c, hipStreamSynchronize(s.st));  // previous batch on this stream released staging
   const size_t bt = (size_t)nt * 4, bn = (size_t)nt * K * 4, bo = (size_t)(n_sent + 1) * 8,
@@ -709,6 +771,19 @@ int w2v_submit_padded(w2v_ctx* c, int32_t stream, const int32_t* sen, const int3
     if (K > 0 && (rc = validate_ids(c, neg + (size_t)b * msl * K, (int64_t)length[b] * K, "neg"))) return rc;
   }
   StreamState& s = c->ss[stream];
+  if (coalescing(c)) {  // the real tokens of each padded sentence join the CSR queue
+    for (int32_t b = 0; b < batch; b++) {
+      const int32_t* t = sen + (size_t)b * msl;
+      s.q_tok.insert(s.q_tok.end(), t, t + length[b]);
+      if (K > 0) {
+        const int32_t* g = neg + (size_t)b * msl * K;
+        s.q_neg.insert(s.q_neg.end(), g, g + (size_t)length[b] * K);
+      }
+      s.q_off.push_back((int64_t)s.q_tok.size());
+    }
+    s.q_alpha.insert(s.q_alpha.end(), alpha, alpha + batch);
+    return queue_full(c, s) ? flush_queue(c, s) : W2V_OK;
+  }
   HIP_TRY(c, hipSetDevice(c->p.device));
   HIP_TRY(c, hipStreamSynchronize(s.st));
   const size_t bt = (size_t)batch * msl * 4, bn = bt * K, bl = (size_t)batch * 4, ba = bl;

@@ -184,6 +184,22 @@ int w2v_train_epoch_graph(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int
 int w2v_train_chunk(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int32_t* tokens,
                     const int64_t* offsets, int64_t n_sentences, int64_t first_token);
 
+/* Coalescing of host batches (Hogwild runs without delta buffers): w2v_submit_csr and
+ * w2v_submit_padded append each batch, as CSR, to a per-stream queue that is launched
+ * once it holds max_sentences sentences or max_tokens tokens, and by w2v_sync /
+ * w2v_sync_all (which every call that reads the model or the counters makes) and
+ * w2v_destroy. One wave trains one sentence, so a launch lasts as long as its longest
+ * sentence and a batch of a few hundred sentences (the reference's -kernel-batch-size
+ * 200) leaves most of the device idle. The caller's buffers are still free for reuse
+ * when a submit returns. Deterministic and delta runs launch every batch as submitted.
+ * max_sentences = 0 turns coalescing off; max_tokens = 0 means no token limit. Defaults
+ * W2V_DEFAULT_COALESCE_SENTENCES / _TOKENS. w2v_sync_all launches every stream's queue,
+ * so it (and the calls that make it) must not run while another thread submits. No
+ * counterpart in the reference, whose driver launches every batch. */
+#define W2V_DEFAULT_COALESCE_SENTENCES 20000
+#define W2V_DEFAULT_COALESCE_TOKENS (1 << 20)
+int w2v_set_coalesce(w2v_ctx* ctx, int64_t max_sentences, int64_t max_tokens);
+
 /* Launch shape of the update kernel in Hogwild mode: waves per workgroup (1..4) and
  * persistent workgroups (0 = as many as fit on the device). Bounds how many sentences
  * train concurrently. (0, 0) restores the default. No counterpart in the reference,

@@ -350,3 +350,57 @@ def test_hogwild_fast_path_one_wave_matches_oracle(orc, w2vmod, sigmoid, hot):
               flush=True)
         passed += e0 <= TOL and e1 <= TOL
     assert passed >= trials - (1 if sigmoid == "table" else 0)
+
+
+@pytest.mark.parametrize("storage,kernel", [("fp32", "plain"), ("bf16", "mfma")])
+def test_coalesced_host_batches_equal_separate_launches(w2vmod, storage, kernel):
+    """Hogwild host batches (CSR and padded) queued per stream and launched together give
+    the same model as one launch per batch: one wave, rows never revisited, so both train
+    the same sentences in the same order. The queue is launched by the sync (1 launch
+    against 5)."""
+    V, d, W, K, msl = 6000, 128, 3, 4, 40
+    rng = np.random.default_rng(71)
+    bf = storage == "bf16"
+    syn0, syn1 = model(V, d, rng, t_scale=1.0 if bf else 0.5)
+    lens = [12, 1, 30, 7, 19, 25, 3, 40, 9, 16]
+    tokens, off, neg = distinct_batch(V, lens, K, rng)
+    # bf16: updates well above the stochastic-rounding noise of the stored rows
+    alpha = np.linspace(0.2, 0.3, len(lens)).astype(np.float32) if bf else \
+        np.linspace(0.02, 0.03, len(lens)).astype(np.float32)
+    batches = [(0, 2), (2, 5), (5, 6), (6, 8), (8, 10)]
+
+    def run(coalesce):
+        tr = w2vmod.Trainer(V, d, W, K, msl=msl, storage=storage, kernel=kernel, sigmoid="exact",
+                            mode="hogwild", seed=3)
+        tr.set_launch_shape(1, 1)
+        tr.set_hot_rows(0)
+        tr.set_coalesce(20000 if coalesce else 0)
+        tr.upload(syn0, syn1)
+        base = tr.download()  # the stored starting point (bf16-rounded for the MFMA kernel)
+        for i, (b0, b1) in enumerate(batches):
+            t0, t1 = off[b0], off[b1]
+            if i % 2 == 0:
+                tr.submit_csr(tokens[t0:t1], off[b0:b1 + 1] - t0, neg[t0 * K:t1 * K], alpha[b0:b1])
+            else:
+                sen = np.zeros((b1 - b0, msl), np.int32)
+                ng = np.zeros((b1 - b0, msl, K), np.int32)
+                for j, s in enumerate(range(b0, b1)):
+                    L = lens[s]
+                    sen[j, :L] = tokens[off[s]:off[s + 1]]
+                    ng[j, :L] = neg[off[s] * K:off[s + 1] * K].reshape(L, K)
+                tr.submit_padded(sen, ng, np.array(lens[b0:b1], np.int32), alpha[b0:b1])
+        _, launches = tr.kernel_time(0)
+        a0, a1 = tr.download()
+        tr.close()
+        return a0, a1, launches, base
+
+    q0, q1, nq, (b0, b1) = run(True)
+    s0, s1, ns, _ = run(False)
+    assert nq == 1 and ns == len(batches)
+    if storage == "fp32":
+        assert np.array_equal(q0, s0) and np.array_equal(q1, s1)
+    else:  # stochastic rounding draws differ between launches: compare the changes
+        for got, ref, base in ((q0, s0, b0), (q1, s1, b1)):
+            dg, dr = got.astype(np.float64) - base, ref.astype(np.float64) - base
+            m = np.abs(dr) > 0
+            assert np.corrcoef(dg[m], dr[m])[0, 1] > 0.99
