@@ -404,3 +404,13 @@ def test_coalesced_host_batches_equal_separate_launches(w2vmod, storage, kernel)
             dg, dr = got.astype(np.float64) - base, ref.astype(np.float64) - base
             m = np.abs(dr) > 0
             assert np.corrcoef(dg[m], dr[m])[0, 1] > 0.99
+
+
+def test_coalesce_rejects_bad_limits(w2vmod):
+    tr = w2vmod.Trainer(100, 16, 2, 1)
+    for args in ((-1, 0), (10, -5), (1 << 31, 0)):
+        with pytest.raises(w2vmod.W2VError):
+            tr.set_coalesce(*args)
+    tr.set_coalesce(0)  # off
+    tr.set_coalesce(500, 0)  # no token limit
+    tr.close()
