@@ -44,6 +44,7 @@ struct Options {
   std::string precision = "fp32", save_ckpt, resume;
   int device_sampling = 0, device = 0, exact_sigmoid = 0;
   long long hot_rows = -1;  // -1: from the vocabulary counts (w2v_auto_hot_rows)
+  int ingest_threads = 0;   // vocabulary and corpus passes over the train file; 0: all cores
   unsigned long long seed = 1;
 };
 
@@ -87,6 +88,8 @@ void usage(const char* prog) {
   printf("\t-exact-sigmoid <0|1>\n\t\tUse 1/(1+exp(-x)) instead of the 1000-entry table\n");
   printf("\t-hot-rows <int>\n\t\tContext rows of the n most frequent words are written back and reloaded\n"
          "\t\tafter every centre (-1: chosen from the word counts; 0: off)\n");
+  printf("\t-ingest-threads <int>\n\t\tThreads for the vocabulary and corpus passes over -train (0: all cores;\n"
+         "\t\tthe result does not depend on it)\n");
   printf("\t-device <int>\n\t-seed <int>\n");
   printf("\t-save-checkpoint <file>\n\t\tWrite model, syn1neg and counters to <file> at every epoch end\n");
   printf("\t-resume <file>\n\t\tContinue from a checkpoint (same vocabulary, sizes and -threads)\n");
@@ -156,6 +159,7 @@ Options parse(int argc, char** argv) {
   num("-device", o.device);
   if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = strtoull(argv[i + 1], nullptr, 10);
   if ((i = arg_pos("-hot-rows", argc, argv)) > 0 && i + 1 < argc) o.hot_rows = strtoll(argv[i + 1], nullptr, 10);
+  num("-ingest-threads", o.ingest_threads);
   if (o.debug > 0) {
     fprintf(stderr, "** FILES & I/O **\n");
     if (!o.do_export && !o.output.empty()) fprintf(stderr, "!!\tProcess will train but will not export a model!\n");
@@ -384,7 +388,9 @@ int main(int argc, char** argv) {
         return 1;
       }
     } else {
-      vocab.learn(f);
+      const double t0 = now_s();
+      vocab.learn(f, o.ingest_threads);
+      if (o.debug > 1) printf("Vocabulary learned in %.2f s\n", now_s() - t0);
     }
     if (o.recount) vocab.recount(f);
     if (!o.save_vocab.empty()) vocab.save(o.save_vocab);
@@ -394,7 +400,9 @@ int main(int argc, char** argv) {
         return 1;
       }
     } else {
-      corpus.build(vocab, f, o.msl, o.cache_eos);
+      const double t0 = now_s();
+      corpus.build(vocab, f, o.msl, o.cache_eos, o.ingest_threads);
+      if (o.debug > 1) printf("Corpus stream built in %.2f s\n", now_s() - t0);
     }
   } else {
     if (o.read_vocab.empty()) {

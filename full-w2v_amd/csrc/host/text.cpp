@@ -11,6 +11,9 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <string_view>
+#include <thread>
+#include <unordered_map>
 
 namespace w2vhost {
 
@@ -130,7 +133,127 @@ void Vocab::reduce() {
   min_reduce++;
 }
 
-void Vocab::learn(const MappedFile& f) {
+std::vector<uint64_t> line_chunks(const MappedFile& f, int n) {
+  std::vector<uint64_t> b(1, 0);
+  for (int i = 1; i < n; i++) {
+    uint64_t p = f.size * (uint64_t)i / (uint64_t)n;
+    if (p <= b.back()) continue;
+    const void* nl = memchr(f.data + p, '\n', f.size - p);
+    if (!nl) break;
+    p = (uint64_t)((const char*)nl - f.data) + 1;
+    if (p < f.size && p > b.back()) b.push_back(p);
+  }
+  b.push_back(f.size);
+  return b;
+}
+
+int ingest_threads(int requested) {
+  if (requested > 0) return requested;
+  const unsigned h = std::thread::hardware_concurrency();
+  return h == 0 ? 1 : (h > 64 ? 64 : (int)h);
+}
+
+namespace {
+// Read-only word -> index lookup for the corpus pass: open addressing over 2^k >= 2V
+// slots of (full 64-bit hash, index), a few MB instead of the 30M-slot build table, so
+// the probes stay in cache; strings are compared only on a full-hash match.
+struct Lookup {
+  std::vector<uint64_t> key;  // hash, 0 = empty (hashes are forced nonzero)
+  std::vector<int32_t> idx;
+  uint64_t mask = 0;
+  static uint64_t h64(const char* w, size_t* len) {
+    uint64_t h = 1469598103934665603ull;  // FNV-1a
+    size_t n = 0;
+    for (; w[n]; n++) h = (h ^ (unsigned char)w[n]) * 1099511628211ull;
+    *len = n;
+    return h | 1;
+  }
+  explicit Lookup(const Vocab& v) {
+    uint64_t cap = 1024;
+    while (cap < 2 * (uint64_t)v.words.size()) cap <<= 1;
+    key.assign(cap, 0);
+    idx.assign(cap, -1);
+    mask = cap - 1;
+    for (size_t i = 0; i < v.words.size(); i++) {
+      size_t n;
+      const uint64_t h = h64(v.words[i].word.c_str(), &n);
+      uint64_t s = h & mask;
+      while (key[s] != 0) s = (s + 1) & mask;
+      key[s] = h;
+      idx[s] = (int32_t)i;
+    }
+  }
+  int find(const Vocab& v, const char* w) const {
+    size_t n;
+    const uint64_t h = h64(w, &n);
+    for (uint64_t s = h & mask;; s = (s + 1) & mask) {
+      if (key[s] == 0) return -1;
+      if (key[s] == h) {
+        const std::string& x = v.words[idx[s]].word;
+        if (x.size() == n && memcmp(x.data(), w, n) == 0) return idx[s];
+      }
+    }
+  }
+};
+
+struct WordStat {
+  uint64_t cn = 0;
+  uint64_t first = 0;  // byte offset of the first occurrence (insertion order)
+};
+}  // namespace
+
+void Vocab::learn(const MappedFile& f, int threads) {
+  threads = ingest_threads(threads);
+  if (threads > 1 && f.size > 0) {
+    const std::vector<uint64_t> b = line_chunks(f, threads);
+    const int n = (int)b.size() - 1;
+    std::vector<std::unordered_map<std::string, WordStat>> maps(n);
+    std::vector<uint64_t> counted(n, 0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < n; t++)
+      pool.emplace_back([&, t]() {
+        char w[kMaxString];
+        auto& m = maps[t];
+        uint64_t c = 0;
+        for (uint64_t pos = b[t]; pos < b[t + 1];) {
+          const uint64_t at = pos;
+          read_word(w, f.data, b[t + 1], &pos);
+          if (pos == f.size) break;  // the word that reaches end of file is not counted
+          c++;
+          auto it = m.find(w);
+          if (it == m.end()) it = m.emplace(w, WordStat{0, at}).first;
+          it->second.cn++;
+        }
+        counted[t] = c;
+      });
+    for (auto& th : pool) th.join();
+    std::unordered_map<std::string, WordStat> all = std::move(maps[0]);
+    for (int t = 1; t < n; t++)
+      for (auto& kv : maps[t]) {
+        auto it = all.find(kv.first);
+        if (it == all.end()) all.emplace(kv.first, kv.second);
+        else it->second.cn += kv.second.cn;  // chunks are in file order: first stays
+      }
+    if (all.size() + 1 <= kVocabHashSize * 0.7) {
+      std::vector<std::pair<uint64_t, const std::string*>> order;
+      order.reserve(all.size());
+      for (auto& kv : all)
+        if (kv.first != "</s>") order.emplace_back(kv.second.first, &kv.first);
+      std::sort(order.begin(), order.end());
+      words.clear();
+      std::fill(hash.begin(), hash.end(), -1);
+      add("</s>");
+      auto eos = all.find("</s>");
+      if (eos != all.end()) words[0].cn = eos->second.cn;
+      for (auto& o : order) words[add(o.second->c_str())].cn = all[*o.second].cn;
+      train_words = 0;
+      for (uint64_t c : counted) train_words += c;
+      sort_and_prune();
+      train_words++;
+      return;
+    }
+    // past the pruning point: the sequential pass reproduces ReduceVocab exactly
+  }
   char w[kMaxString];
   train_words = 0;
   words.clear();
@@ -187,7 +310,44 @@ bool Vocab::save(const std::string& path) const {
   return true;
 }
 
-void Corpus::build(const Vocab& v, const MappedFile& f, int msl, bool cache_eos) {
+void Corpus::build(const Vocab& v, const MappedFile& f, int msl, bool cache_eos, int threads) {
+  threads = ingest_threads(threads);
+  if (threads > 1 && f.size > 0) {
+    const std::vector<uint64_t> b = line_chunks(f, threads);
+    const int n = (int)b.size() - 1;
+    const Lookup lut(v);
+    std::vector<std::vector<int32_t>> ids(n);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < n; t++)
+      pool.emplace_back([&, t]() {
+        char w[kMaxString];
+        auto& out = ids[t];
+        out.reserve((b[t + 1] - b[t]) / 4);
+        for (uint64_t pos = b[t]; pos < b[t + 1];) {
+          read_word(w, f.data, b[t + 1], &pos);
+          const int idx = lut.find(v, w);
+          if (idx >= 0) out.push_back(idx);  // unknown words are skipped
+        }
+      });
+    for (auto& th : pool) th.join();
+    stream.clear();
+    start.clear();
+    size_t total = 0;
+    for (auto& x : ids) total += x.size();
+    stream.reserve(total + 1);
+    uint64_t len = 0;
+    for (auto& x : ids)
+      for (const int32_t idx : x)
+        if (cache_eos || idx != 0) {
+          if (len == 0) start.push_back((int64_t)stream.size());
+          stream.push_back(idx);
+          len++;
+          if (idx == 0 || len >= (uint64_t)msl) len = 0;
+        }
+    stream.push_back(0);
+    train_words = v.train_words;
+    return;
+  }
   char w[kMaxString];
   stream.clear();
   start.clear();

@@ -25,6 +25,12 @@ struct MappedFile {
 // kMaxString-1 bytes are truncated.
 void read_word(char* word, const char* buf, uint64_t size, uint64_t* pos);
 
+// Byte ranges of up to n chunks of whole lines covering the file: every boundary is just
+// after a newline, so the word reader yields the same tokens chunk by chunk as over the
+// whole file.
+std::vector<uint64_t> line_chunks(const MappedFile& f, int n);
+int ingest_threads(int requested);
+
 struct Vocab {
   struct Entry {
     std::string word;
@@ -43,8 +49,11 @@ struct Vocab {
   void sort_and_prune();  // SortVocab (legacy:74-95): </s> stays first, cn < min_count dropped
   void reduce();          // ReduceVocab (legacy:98-118)
   void rehash();
-  // LearnVocabFromTrainFile (improved:136-164)
-  void learn(const MappedFile& f);
+  // LearnVocabFromTrainFile (improved:136-164). threads > 1 counts chunks of whole lines
+  // in parallel and rebuilds the sequential insertion order from first occurrences, so
+  // the result is identical; past the reference's pruning point (ReduceVocab) it falls
+  // back to the sequential pass. threads <= 0: one per hardware thread.
+  void learn(const MappedFile& f, int threads = 1);
   // ReadVocab (improved:167-196); returns false if the file cannot be opened
   bool read(const std::string& path);
   // recountVocab (improved:199-223)
@@ -59,8 +68,9 @@ struct Corpus {
   std::vector<int32_t> stream;
   std::vector<int64_t> start;
   uint64_t train_words = 0;
-  // createGlobalStream (improved:244-281)
-  void build(const Vocab& v, const MappedFile& f, int msl, bool cache_eos);
+  // createGlobalStream (improved:244-281); threads > 1 maps chunks of whole lines to ids
+  // in parallel, then splits sentences over the joined stream (identical result)
+  void build(const Vocab& v, const MappedFile& f, int msl, bool cache_eos, int threads = 1);
   // readCorpusCache (improved:320-401); false if the file cannot be opened
   bool read_cache(const std::string& path, int msl, bool cache_eos);
   // saveCorpusCache (improved:306-318) and saveCorpus (improved:293-304)

@@ -76,6 +76,39 @@ def test_help_exits_zero(corpus_dir):
     assert r.returncode == 0 and "-kernel-batch-size" in r.stdout and "-precision" in r.stdout
 
 
+def tricky_text(path, n_lines=4000, seed=5):
+    """CRLF and LF lines, tabs and runs of spaces, empty lines, an over-long word, rare
+    words below -min-count, and no newline at the end of the file."""
+    rng = np.random.default_rng(seed)
+    p = 1.0 / np.arange(1, 401) ** 1.1
+    p /= p.sum()
+    lines = []
+    for i in range(n_lines):
+        ws = [f"w{j}" for j in rng.choice(400, int(rng.integers(0, 25)), p=p)]
+        if i % 97 == 0:
+            ws.append("x" * 1200)
+        if i % 13 == 0:
+            ws.append(f"rare{i}")
+        sep = "\t" if i % 7 == 0 else ("   " if i % 5 == 0 else " ")
+        lines.append(sep.join(ws) + ("\r" if i % 3 == 0 else ""))
+    path.write_bytes(("\n".join(lines)).encode())
+
+
+@pytest.mark.parametrize("eos", [True, False])
+def test_parallel_ingest_equals_sequential(tmp_path, eos):
+    """Vocabulary and corpus stream built on 1 and on 5 threads (chunks of whole lines)
+    are byte-identical: same words, counts, order and sentences."""
+    tricky_text(tmp_path / "c.txt")
+    out = {}
+    for t in (1, 5):
+        args = ["-train", "c.txt", "-save-vocab", f"v{t}.txt", "-save-corpus-cache", f"k{t}.txt",
+                "-min-count", "3", "-msl", "7", "-ingest-threads", str(t)] + (["-cache-eos"] if eos else [])
+        run(args, tmp_path, check=0)
+        out[t] = ((tmp_path / f"v{t}.txt").read_bytes(), (tmp_path / f"k{t}.txt").read_bytes())
+    assert len(out[1][0]) > 100 and len(out[1][1]) > 1000
+    assert out[1] == out[5]
+
+
 def words_text(tmp_path, n_lines=3000, seed=0):
     rng = np.random.default_rng(seed)
     vocab = [f"w{i}" for i in range(300)]
