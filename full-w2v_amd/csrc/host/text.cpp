@@ -200,6 +200,52 @@ struct WordStat {
   uint64_t cn = 0;
   uint64_t first = 0;  // byte offset of the first occurrence (insertion order)
 };
+
+// Per-thread word counts: open addressing keyed by the 64-bit hash, word bytes in an
+// arena (no allocation per word), grown at half load.
+struct CountTable {
+  struct Slot {
+    uint64_t h = 0;  // 0 = empty
+    uint32_t off = 0, len = 0;
+    WordStat st;
+  };
+  std::vector<Slot> slots = std::vector<Slot>(1 << 16);
+  std::string arena;
+  size_t used = 0;
+  void grow() {
+    std::vector<Slot> old(slots.size() * 2);
+    old.swap(slots);
+    const uint64_t mask = slots.size() - 1;
+    for (const Slot& x : old)
+      if (x.h) {
+        uint64_t i = x.h & mask;
+        while (slots[i].h) i = (i + 1) & mask;
+        slots[i] = x;
+      }
+  }
+  void add(const char* w, uint64_t at) {
+    size_t n;
+    const uint64_t h = Lookup::h64(w, &n);
+    uint64_t mask = slots.size() - 1;
+    for (uint64_t i = h & mask;; i = (i + 1) & mask) {
+      Slot& x = slots[i];
+      if (x.h == 0) {
+        x.h = h;
+        x.off = (uint32_t)arena.size();
+        x.len = (uint32_t)n;
+        arena.append(w, n);
+        x.st.cn = 1;
+        x.st.first = at;
+        if (++used * 2 > slots.size()) grow();
+        return;
+      }
+      if (x.h == h && x.len == n && memcmp(arena.data() + x.off, w, n) == 0) {
+        x.st.cn++;
+        return;
+      }
+    }
+  }
+};
 }  // namespace
 
 void Vocab::learn(const MappedFile& f, int threads) {
@@ -207,33 +253,32 @@ void Vocab::learn(const MappedFile& f, int threads) {
   if (threads > 1 && f.size > 0) {
     const std::vector<uint64_t> b = line_chunks(f, threads);
     const int n = (int)b.size() - 1;
-    std::vector<std::unordered_map<std::string, WordStat>> maps(n);
+    std::vector<CountTable> tabs(n);
     std::vector<uint64_t> counted(n, 0);
     std::vector<std::thread> pool;
     for (int t = 0; t < n; t++)
       pool.emplace_back([&, t]() {
         char w[kMaxString];
-        auto& m = maps[t];
+        CountTable& m = tabs[t];
         uint64_t c = 0;
         for (uint64_t pos = b[t]; pos < b[t + 1];) {
           const uint64_t at = pos;
           read_word(w, f.data, b[t + 1], &pos);
           if (pos == f.size) break;  // the word that reaches end of file is not counted
           c++;
-          auto it = m.find(w);
-          if (it == m.end()) it = m.emplace(w, WordStat{0, at}).first;
-          it->second.cn++;
+          m.add(w, at);
         }
         counted[t] = c;
       });
     for (auto& th : pool) th.join();
-    std::unordered_map<std::string, WordStat> all = std::move(maps[0]);
-    for (int t = 1; t < n; t++)
-      for (auto& kv : maps[t]) {
-        auto it = all.find(kv.first);
-        if (it == all.end()) all.emplace(kv.first, kv.second);
-        else it->second.cn += kv.second.cn;  // chunks are in file order: first stays
-      }
+    std::unordered_map<std::string, WordStat> all;
+    all.reserve(tabs[0].used * 2);
+    for (int t = 0; t < n; t++)  // chunks are in file order: the first offset stays
+      for (const auto& x : tabs[t].slots)
+        if (x.h) {
+          auto r = all.emplace(std::string(tabs[t].arena.data() + x.off, x.len), x.st);
+          if (!r.second) r.first->second.cn += x.st.cn;
+        }
     if (all.size() + 1 <= kVocabHashSize * 0.7) {
       std::vector<std::pair<uint64_t, const std::string*>> order;
       order.reserve(all.size());
@@ -330,21 +375,40 @@ void Corpus::build(const Vocab& v, const MappedFile& f, int msl, bool cache_eos,
         }
       });
     for (auto& th : pool) th.join();
-    stream.clear();
-    start.clear();
-    size_t total = 0;
-    for (auto& x : ids) total += x.size();
-    stream.reserve(total + 1);
-    uint64_t len = 0;
-    for (auto& x : ids)
-      for (const int32_t idx : x)
-        if (cache_eos || idx != 0) {
-          if (len == 0) start.push_back((int64_t)stream.size());
-          stream.push_back(idx);
+    // Join in parallel. Each chunk's place in the stream follows from the kept-token
+    // counts before it. The sentence-length counter at a chunk's first token is known
+    // too: with cache_eos every chunk begins right after a </s> (counter 0); without it
+    // </s> is dropped and sentences are cut every msl tokens, so the counter is the
+    // stream position mod msl.
+    std::vector<uint64_t> kept(n + 1, 0);
+    for (int t = 0; t < n; t++) {
+      uint64_t k = 0;
+      for (const int32_t idx : ids[t]) k += (cache_eos || idx != 0);
+      kept[t + 1] = kept[t] + k;
+    }
+    stream.assign(kept[n] + 1, 0);  // the stream always ends with </s>
+    std::vector<std::vector<int64_t>> starts(n);
+    pool.clear();
+    for (int t = 0; t < n; t++)
+      pool.emplace_back([&, t]() {
+        uint64_t pos = kept[t];
+        uint64_t len = cache_eos ? 0 : pos % (uint64_t)msl;
+        auto& st = starts[t];
+        for (const int32_t idx : ids[t]) {
+          if (!cache_eos && idx == 0) continue;
+          if (len == 0) st.push_back((int64_t)pos);
+          stream[pos++] = idx;
           len++;
           if (idx == 0 || len >= (uint64_t)msl) len = 0;
         }
-    stream.push_back(0);
+        std::vector<int32_t>().swap(ids[t]);
+      });
+    for (auto& th : pool) th.join();
+    start.clear();
+    size_t ns = 0;
+    for (auto& x : starts) ns += x.size();
+    start.reserve(ns);
+    for (auto& x : starts) start.insert(start.end(), x.begin(), x.end());
     train_words = v.train_words;
     return;
   }
