@@ -298,7 +298,8 @@ int w2v_create(const w2v_params* params, w2v_ctx** out) {
   if (p.negative < 0 || p.negative > 64)
     return fail(nullptr, W2V_ERR_UNSUPPORTED, "negative %d outside 0..64", p.negative);
   if (p.max_sentence_length < 1) return fail(nullptr, W2V_ERR_ARG, "max_sentence_length must be >= 1");
-  if (p.streams < 1 || p.streams > 64) return fail(nullptr, W2V_ERR_ARG, "streams must be in 1..64");
+  if (p.streams < 1 || p.streams > W2V_MAX_STREAMS)
+    return fail(nullptr, W2V_ERR_ARG, "streams must be in 1..%d", W2V_MAX_STREAMS);
   if (!((p.kernel == W2V_KERNEL_PLAIN && p.storage == W2V_STORAGE_FP32) ||
         (p.kernel == W2V_KERNEL_MFMA && p.storage == W2V_STORAGE_BF16)))
     return fail(nullptr, W2V_ERR_UNSUPPORTED,

@@ -431,8 +431,8 @@ int main(int argc, char** argv) {
     printf("%lld sentences spread across %d CPU threads == %lld sentences/thread\n", (long long)n_sent,
            o.threads, (long long)(n_sent / std::max(1, o.threads)));
   if (o.output.empty()) return 0;
-  if (o.threads < 1 || o.streams < 1 || o.threads * o.streams > 64) {
-    fprintf(stderr, "-threads x -streams must be between 1 and 64\n");
+  if (o.threads < 1 || o.streams < 1 || o.threads * o.streams > W2V_MAX_STREAMS) {
+    fprintf(stderr, "-threads x -streams must be between 1 and %d\n", W2V_MAX_STREAMS);
     return 1;
   }
 

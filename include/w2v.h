@@ -27,6 +27,8 @@
 extern "C" {
 #endif
 
+#define W2V_MAX_STREAMS 1024 /* the reference's sweeps use up to 40 threads x 4 streams */
+
 #define W2V_OK 0
 #define W2V_ERR_ARG (-1)         /* invalid argument */
 #define W2V_ERR_HIP (-2)         /* HIP runtime failure (message has the HIP error) */
@@ -60,7 +62,7 @@ typedef struct w2v_params {
   int32_t sigmoid;             /* W2V_SIGMOID_* */
   int32_t mode;                /* W2V_MODE_* */
   int32_t device;              /* HIP device ordinal */
-  int32_t streams;             /* submission streams, >= 1 (reference: -streams) */
+  int32_t streams;             /* submission streams, 1..W2V_MAX_STREAMS (reference: -threads x -streams) */
   int32_t reserved;
   uint64_t seed;               /* device sampling and stochastic-rounding seed */
 } w2v_params;

@@ -168,3 +168,16 @@ def test_cli_binary_export(tmp_path):
         assert np.isfinite(vec).all()
         pos = sp + 1 + 4 * d + 1
     assert pos == len(rest)
+
+
+@pytest.mark.gpu
+def test_cli_reference_sweep_streams(tmp_path):
+    """The reference's replication sweep runs -threads 40 -streams 4 (160 streams)."""
+    words_text(tmp_path, 2000)
+    r = run(["-train", "big.txt", "-output", "vec.txt", "-size", "64", "-window", "3", "-negative", "4",
+             "-iter", "1", "-threads", "40", "-streams", "4", "-min-count", "1", "-sample", "1e-3",
+             "-precision", "bf16"], tmp_path, check=0)
+    wps, words, secs = parse_contract(r.stdout)
+    assert words > 0 and wps > 0
+    txt = (tmp_path / "vec.txt").read_text().splitlines()
+    assert len(txt) == int(txt[0].split()[0]) + 1
