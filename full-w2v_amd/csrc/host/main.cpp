@@ -35,6 +35,7 @@ constexpr int64_t kBatchTokens = (int64_t)1 << 20;  // token cap of one host bat
 
 struct Options {
   std::string output, save_vocab, read_vocab, train, save_corpus, save_cache, read_cache;
+  std::string save_bin, read_bin;  // binary corpus cache
   int binary = 0, debug = 2;
   bool recount = false, do_export = true, cache_eos = false, pre_shuffle = false, epoch_shuffle = false;
   int negative = 5, min_count = 5, threads = 12, streams = 4, epochs = 5, window = 5,
@@ -67,6 +68,7 @@ void usage(const char* prog) {
   printf("\t-min-count <int>\n\t\tDiscard words that appear less than <int> times; default is 5\n");
   printf("\t-recount\n\t\tReconstruct vocabulary counts from training data\n");
   printf("\t-save-corpus <file>\n\t-save-corpus-cache <file>\n\t-read-corpus-cache <file>\n");
+  printf("\t-save-corpus-bin <file>\n\t-read-corpus-bin <file>\n\t\tBinary corpus cache (token ids as stored; read at memory speed)\n");
   printf("\t-cache-eos\n\t\tInclude cached EOS tokens from corpus paragraph delimiters\n");
   printf("\t-pre-shuffle\n\t-epoch-shuffle\n");
   printf("** TRAINING PARAMETERS **\n");
@@ -122,6 +124,8 @@ Options parse(int argc, char** argv) {
   str("-save-corpus", o.save_corpus);
   str("-save-corpus-cache", o.save_cache);
   str("-read-corpus-cache", o.read_cache);
+  str("-save-corpus-bin", o.save_bin);
+  str("-read-corpus-bin", o.read_bin);
   if (arg_pos("-cache-eos", argc, argv) > 0) o.cache_eos = true;
   if (arg_pos("-pre-shuffle", argc, argv) > 0) o.pre_shuffle = true;
   if (arg_pos("-epoch-shuffle", argc, argv) > 0) o.epoch_shuffle = true;
@@ -369,6 +373,17 @@ void* train_thread(void* p) {
 
 }  // namespace
 
+bool read_corpus_bin(const Options& o, const Vocab& vocab, Corpus& corpus) {
+  std::string err;
+  const double t0 = now_s();
+  if (!corpus.read_bin(o.read_bin, o.msl, o.cache_eos, (int64_t)vocab.words.size(), &err)) {
+    printf("Binary corpus cache \"%s\": %s\n", o.read_bin.c_str(), err.c_str());
+    return false;
+  }
+  if (o.debug > 1) printf("Binary corpus cache read in %.2f s\n", now_s() - t0);
+  return true;
+}
+
 int main(int argc, char** argv) {
   Options o = parse(argc, argv);
   Vocab vocab;
@@ -394,7 +409,9 @@ int main(int argc, char** argv) {
     }
     if (o.recount) vocab.recount(f);
     if (!o.save_vocab.empty()) vocab.save(o.save_vocab);
-    if (!o.read_cache.empty()) {
+    if (!o.read_bin.empty()) {
+      if (!read_corpus_bin(o, vocab, corpus)) return 1;
+    } else if (!o.read_cache.empty()) {
       if (!corpus.read_cache(o.read_cache, o.msl, o.cache_eos)) {
         printf("Corpus cache file \"%s\" not found!\n", o.read_cache.c_str());
         return 1;
@@ -414,17 +431,25 @@ int main(int argc, char** argv) {
       return 1;
     }
     if (!o.save_vocab.empty()) vocab.save(o.save_vocab);
-    if (o.read_cache.empty()) {
-      printf("ERROR: No corpus cache or training file to learn from! Please specify -train or -read-corpus-cache\n");
-      return 1;
-    }
-    if (!corpus.read_cache(o.read_cache, o.msl, o.cache_eos)) {
-      printf("Corpus cache file \"%s\" not found!\n", o.read_cache.c_str());
-      return 1;
+    if (!o.read_bin.empty()) {
+      if (!read_corpus_bin(o, vocab, corpus)) return 1;
+    } else {
+      if (o.read_cache.empty()) {
+        printf("ERROR: No corpus cache or training file to learn from! Please specify -train or -read-corpus-cache\n");
+        return 1;
+      }
+      if (!corpus.read_cache(o.read_cache, o.msl, o.cache_eos)) {
+        printf("Corpus cache file \"%s\" not found!\n", o.read_cache.c_str());
+        return 1;
+      }
     }
   }
   if (!corpus.train_words) corpus.train_words = vocab.train_words;
   if (!o.save_cache.empty()) corpus.save_cache(o.save_cache, o.msl);
+  if (!o.save_bin.empty() && !corpus.save_bin(o.save_bin, o.msl, o.cache_eos, (int64_t)vocab.words.size())) {
+    printf("Cannot write binary corpus cache \"%s\"\n", o.save_bin.c_str());
+    return 1;
+  }
   if (!o.save_corpus.empty()) corpus.save_text(o.save_corpus, vocab, o.msl);
   const int64_t n_sent = (int64_t)corpus.start.size();
   if (o.debug > 1)

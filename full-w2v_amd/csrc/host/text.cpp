@@ -500,6 +500,78 @@ bool Corpus::save_text(const std::string& path, const Vocab& v, int msl) const {
   return true;
 }
 
+namespace {
+struct BinHeader {
+  char magic[8];
+  uint64_t n_tokens, n_sent, train_words;
+  int32_t msl, cache_eos;
+  int64_t vocab_size;
+};
+constexpr char kBinMagic[8] = {'W', '2', 'V', 'C', 'O', 'R', 'P', '1'};
+}  // namespace
+
+bool Corpus::save_bin(const std::string& path, int msl, bool cache_eos, int64_t vocab_size) const {
+  FILE* fo = fopen(path.c_str(), "wb");
+  if (!fo) return false;
+  BinHeader h{};
+  memcpy(h.magic, kBinMagic, 8);
+  h.n_tokens = stream.size();
+  h.n_sent = start.size();
+  h.train_words = train_words;
+  h.msl = msl;
+  h.cache_eos = cache_eos ? 1 : 0;
+  h.vocab_size = vocab_size;
+  bool ok = fwrite(&h, sizeof(h), 1, fo) == 1;
+  ok = ok && fwrite(stream.data(), 4, stream.size(), fo) == stream.size();
+  ok = ok && fwrite(start.data(), 8, start.size(), fo) == start.size();
+  return fclose(fo) == 0 && ok;
+}
+
+bool Corpus::read_bin(const std::string& path, int msl, bool cache_eos, int64_t vocab_size, std::string* err) {
+  MappedFile f;
+  if (!f.open(path)) {
+    *err = "cannot open";
+    return false;
+  }
+  BinHeader h;
+  if (f.size < sizeof(h)) {
+    *err = "too short for a header";
+    return false;
+  }
+  memcpy(&h, f.data, sizeof(h));
+  if (memcmp(h.magic, kBinMagic, 8) != 0) {
+    *err = "not a binary corpus cache";
+    return false;
+  }
+  if (f.size != sizeof(h) + h.n_tokens * 4 + h.n_sent * 8) {
+    *err = "size does not match its header";
+    return false;
+  }
+  if (h.msl != msl || (h.cache_eos != 0) != cache_eos || h.vocab_size != vocab_size) {
+    char b[256];
+    snprintf(b, sizeof(b), "built with -msl %d, -cache-eos %d and %lld words; this run has -msl %d, -cache-eos %d and %lld",
+             h.msl, h.cache_eos, (long long)h.vocab_size, msl, cache_eos ? 1 : 0, (long long)vocab_size);
+    *err = b;
+    return false;
+  }
+  stream.resize(h.n_tokens);
+  start.resize(h.n_sent);
+  memcpy(stream.data(), f.data + sizeof(h), h.n_tokens * 4);
+  memcpy(start.data(), f.data + sizeof(h) + h.n_tokens * 4, h.n_sent * 8);
+  for (const int32_t t : stream)
+    if (t < 0 || t >= vocab_size) {
+      *err = "token id outside the vocabulary";
+      return false;
+    }
+  for (const int64_t s : start)
+    if (s < 0 || (uint64_t)s >= h.n_tokens) {
+      *err = "sentence start outside the stream";
+      return false;
+    }
+  train_words = h.train_words;
+  return true;
+}
+
 void Corpus::shuffle() {
   if (start.empty()) return;
   for (uint64_t i = start.size() - 1; i > 0; i--) {

@@ -78,6 +78,12 @@ struct Corpus {
   bool save_text(const std::string& path, const Vocab& v, int msl) const;
   // shuffleSentences (improved:283-291)
   void shuffle();
+  // Binary corpus cache (no counterpart in the reference): a header, then the int32
+  // stream and the int64 sentence starts as stored in memory, so a later run maps them
+  // at memory speed instead of parsing text. The header records -msl, -cache-eos and the
+  // vocabulary size; read_bin rejects a file built with other values (message in *err).
+  bool save_bin(const std::string& path, int msl, bool cache_eos, int64_t vocab_size) const;
+  bool read_bin(const std::string& path, int msl, bool cache_eos, int64_t vocab_size, std::string* err);
 };
 
 // saveModel (improved:225-242): "V d\n" then "word " + d values (text "%f " or raw

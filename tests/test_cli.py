@@ -181,3 +181,24 @@ def test_cli_reference_sweep_streams(tmp_path):
     assert words > 0 and wps > 0
     txt = (tmp_path / "vec.txt").read_text().splitlines()
     assert len(txt) == int(txt[0].split()[0]) + 1
+
+
+def test_binary_corpus_cache_round_trip(tmp_path):
+    """-save-corpus-bin then -read-corpus-bin gives the same corpus (compared through the
+    reference's text cache); other -msl / -cache-eos values and a damaged file are refused."""
+    tricky_text(tmp_path / "c.txt")
+    base = ["-min-count", "3", "-msl", "7", "-cache-eos"]
+    run(["-train", "c.txt", "-save-vocab", "v.txt", "-save-corpus-cache", "k1.txt", "-save-corpus-bin", "b.bin"]
+        + base, tmp_path, check=0)
+    run(["-read-vocab", "v.txt", "-read-corpus-bin", "b.bin", "-save-corpus-cache", "k2.txt"] + base, tmp_path,
+        check=0)
+    assert (tmp_path / "k1.txt").read_bytes() == (tmp_path / "k2.txt").read_bytes()
+    r = run(["-read-vocab", "v.txt", "-read-corpus-bin", "b.bin", "-min-count", "3", "-msl", "9", "-cache-eos"],
+            tmp_path, check=1)
+    assert "built with -msl 7" in r.stdout
+    r = run(["-read-vocab", "v.txt", "-read-corpus-bin", "b.bin", "-min-count", "3", "-msl", "7"], tmp_path, check=1)
+    assert "-cache-eos 1" in r.stdout
+    raw = (tmp_path / "b.bin").read_bytes()
+    (tmp_path / "t.bin").write_bytes(raw[:-5])
+    r = run(["-read-vocab", "v.txt", "-read-corpus-bin", "t.bin"] + base, tmp_path, check=1)
+    assert "size does not match" in r.stdout
