@@ -468,6 +468,30 @@ int w2v_sigmoid_tables(float* exp_table, float* nexp_table) {
   return W2V_OK;
 }
 
+int w2v_host_alloc(size_t bytes, void** out) {
+  if (!out) return fail(nullptr, W2V_ERR_ARG, "out is required");
+  *out = nullptr;
+  if (bytes == 0) return fail(nullptr, W2V_ERR_ARG, "bytes must be positive");
+  void* p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(nullptr, W2V_ERR_NOMEM, "hipHostMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
+  }
+  *out = p;
+  return W2V_OK;
+}
+
+int w2v_host_free(void* p) {
+  if (!p) return W2V_OK;
+  hipError_t e = hipHostFree(p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(nullptr, W2V_ERR_HIP, "hipHostFree: %s", hipGetErrorString(e));
+  }
+  return W2V_OK;
+}
+
 int w2v_sync_all(w2v_ctx* c) {
   if (!c) return W2V_ERR_ARG;
   HIP_TRY(c, hipSetDevice(c->p.device));

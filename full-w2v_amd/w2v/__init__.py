@@ -24,7 +24,7 @@ MODE = {"hogwild": 0, "deterministic": 1}
 # Every function declared in include/w2v.h, in declaration order.
 EXPORTS = [
     "w2v_create", "w2v_destroy", "w2v_last_error", "w2v_model_upload", "w2v_model_download",
-    "w2v_checkpoint_save", "w2v_checkpoint_load", "w2v_checkpoint_read_header", "w2v_sigmoid_tables", "w2v_submit_padded", "w2v_submit_csr", "w2v_sync", "w2v_sync_all",
+    "w2v_checkpoint_save", "w2v_checkpoint_load", "w2v_checkpoint_read_header", "w2v_sigmoid_tables", "w2v_submit_padded", "w2v_submit_csr", "w2v_host_alloc", "w2v_host_free", "w2v_sync", "w2v_sync_all",
     "w2v_set_delta_mode", "w2v_delta_download", "w2v_tables_upload", "w2v_set_schedule",
     "w2v_corpus_upload", "w2v_train_device", "w2v_train_epoch_graph", "w2v_train_chunk", "w2v_get_stats", "w2v_heldout_loss",
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
@@ -99,6 +99,8 @@ def lib() -> ctypes.CDLL:
             "w2v_sigmoid_tables": [vp, vp],
             "w2v_submit_padded": [vp, i32, vp, vp, vp, vp, i32],
             "w2v_submit_csr": [vp, i32, vp, vp, i64, vp, vp],
+            "w2v_host_alloc": [ctypes.c_size_t, ctypes.POINTER(vp)],
+            "w2v_host_free": [vp],
             "w2v_sync": [vp, i32],
             "w2v_sync_all": [vp],
             "w2v_set_delta_mode": [vp, i32],
@@ -202,6 +204,29 @@ def synth_cluster_of(seed, clusters, words):
 def synth_config(clusters=64, len_median=22, len_max=1000, zipf_s=1.0, stay=0.9, in_cluster=0.8,
                  len_sigma=1.0, seed=1):
     return Synth(clusters, len_median, len_max, 0, zipf_s, stay, in_cluster, len_sigma, seed)
+
+
+class PinnedArray:
+    """A numpy view of page-locked host memory from w2v_host_alloc; close() frees it."""
+
+    def __init__(self, shape, dtype):
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        p = ctypes.c_void_p()
+        rc = lib().w2v_host_alloc(n, ctypes.byref(p))
+        if rc:
+            raise W2VError(rc, lib().w2v_last_error(None).decode())
+        self._p = p
+        buf = (ctypes.c_char * n).from_address(p.value)
+        self.array = np.frombuffer(buf, dt).reshape(shape)
+
+    def close(self):
+        if self._p:
+            self.array = None
+            rc = lib().w2v_host_free(self._p)
+            self._p = None
+            if rc:
+                raise W2VError(rc, lib().w2v_last_error(None).decode())
 
 
 def sigmoid_tables():

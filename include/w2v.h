@@ -132,6 +132,13 @@ int w2v_submit_padded(w2v_ctx* ctx, int32_t stream, const int32_t* sen, const in
  * its negatives neg[(offsets[s]+w)*K + k], learning rate alpha[s]. */
 int w2v_submit_csr(w2v_ctx* ctx, int32_t stream, const int32_t* tokens, const int64_t* offsets,
                    int64_t n_sentences, const int32_t* neg, const float* alpha);
+/* Page-locked host memory for batch buffers: replaces the driver's cudaMallocHost
+ * (FULL-W2V-driver.cu:144-152) and cudaFreeHost (driver.cu:328-331). Submissions copy the
+ * caller's buffers before returning, so any host memory works; this is for callers that
+ * keep the reference's pinned buffers. hipHostMalloc with default flags; *out is NULL on
+ * error, and w2v_host_free(NULL) is a no-op. */
+int w2v_host_alloc(size_t bytes, void** out);
+int w2v_host_free(void* p);
 /* Replaces cudaStreamSynchronize on one submission stream (driver.cu:286). */
 int w2v_sync(w2v_ctx* ctx, int32_t stream);
 /* Replaces the final cudaDeviceSynchronize (driver.cu:323). */

@@ -53,3 +53,15 @@ def test_create_validates_before_touching_the_device(w2vmod, kw, code):
     else:
         tr.close()  # a GPU is present and the arguments were valid
         assert code is None
+
+
+def test_host_alloc_argument_checks(w2vmod):
+    """Argument errors are reported before any HIP call (no device needed)."""
+    import ctypes
+
+    lib = w2vmod.lib()
+    p = ctypes.c_void_p(1)
+    assert lib.w2v_host_alloc(0, ctypes.byref(p)) == -1 and p.value is None
+    assert b"positive" in lib.w2v_last_error(None)
+    assert lib.w2v_host_alloc(16, None) == -1
+    assert lib.w2v_host_free(None) == 0

@@ -113,6 +113,41 @@ def test_padded_equals_csr(orc, w2vmod):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+def test_padded_from_pinned_buffers(orc, w2vmod):
+    """The reference driver's batch buffers are page-locked (cudaMallocHost,
+    driver.cu:144-152); the same padded batch submitted from w2v_host_alloc memory gives the
+    same deltas as the CSR submission, and the buffers are freed without error."""
+    V, d, W, K = 50, 128, 5, 5
+    rng = np.random.default_rng(3)
+    syn0, syn1 = model(V, d, rng)
+    lens = [13, 4, 29]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(0, V, off[-1]).astype(np.int32)
+    neg = rng.integers(0, V, off[-1] * K).astype(np.int32)
+    alpha = np.array([0.025, 0.02, 0.01], np.float32)
+    a = gpu_delta(w2vmod, V, d, W, K, syn0, syn1, tokens, off, neg, alpha)
+    msl = max(lens)
+    bufs = [w2vmod.PinnedArray((3, msl), np.int32), w2vmod.PinnedArray((3, msl, K), np.int32),
+            w2vmod.PinnedArray((3,), np.int32), w2vmod.PinnedArray((3,), np.float32)]
+    sen, ng, ln, al = (b.array for b in bufs)
+    sen[:] = 0
+    ng[:] = 0
+    ln[:] = lens
+    al[:] = alpha
+    for s in range(3):
+        sen[s, :lens[s]] = tokens[off[s]:off[s + 1]]
+        ng[s, :lens[s]] = neg[off[s] * K:off[s + 1] * K].reshape(-1, K)
+    tr = w2vmod.Trainer(V, d, W, K, msl=msl, sigmoid="exact", mode="deterministic")
+    tr.upload(syn0, syn1)
+    tr.set_delta_mode(True)
+    tr.submit_padded(sen, ng, ln, al)
+    b = tr.delta_download()
+    tr.close()
+    for buf in bufs:
+        buf.close()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
 def test_table_sigmoid_matches_oracle_table(orc, w2vmod):
     """Table sigmoid: bucket-for-bucket agreement except rare FP32/FP64 edge straddles."""
     V, d, W, K = 60, 128, 5, 5
