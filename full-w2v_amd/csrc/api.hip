@@ -933,7 +933,7 @@ static int check_device_training(w2v_ctx* c, int32_t stream, int32_t epoch) {
   if (!c->d_corpus) return fail(c, W2V_ERR_STATE, "no device corpus uploaded");
   if (c->p.negative > 0 && !c->d_unigram) return fail(c, W2V_ERR_STATE, "no unigram table uploaded");
   if (epoch < 0 || epoch >= c->epochs) return fail(c, W2V_ERR_ARG, "epoch outside the schedule");
-  return W2V_OK;
+  return flush_queue(c, c->ss[stream]);  // host batches queued on this stream go first
 }
 
 int w2v_train_device(w2v_ctx* c, int32_t stream, int32_t epoch, int64_t sb, int64_t se) {
@@ -1049,6 +1049,7 @@ int w2v_train_chunk(w2v_ctx* c, int32_t stream, int32_t epoch, const int32_t* to
   if (offsets[0] != 0) return fail(c, W2V_ERR_ARG, "offsets[0] must be 0");
   for (int64_t s = 0; s < ns; s++)
     if (offsets[s + 1] < offsets[s]) return fail(c, W2V_ERR_ARG, "offsets must be non-decreasing");
+  if ((rc = flush_queue(c, c->ss[stream]))) return rc;  // host batches queued on this stream go first
   const int64_t n = offsets[ns];
   if (n > 0 && !tokens) return fail(c, W2V_ERR_ARG, "tokens required");
   if (n >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "at most 2^31-2 tokens per chunk");

@@ -201,6 +201,8 @@ int w2v_train_chunk(w2v_ctx* ctx, int32_t stream, int32_t epoch, const int32_t* 
  * sentence and a batch of a few hundred sentences (the reference's -kernel-batch-size
  * 200) leaves most of the device idle. The caller's buffers are still free for reuse
  * when a submit returns. Deterministic and delta runs launch every batch as submitted.
+ * Device-sampling calls on a stream (w2v_train_device, w2v_train_epoch_graph,
+ * w2v_train_chunk) launch that stream's queue first, so stream order is kept.
  * max_sentences = 0 turns coalescing off; max_tokens = 0 means no token limit. Defaults
  * W2V_DEFAULT_COALESCE_SENTENCES / _TOKENS. w2v_sync_all launches every stream's queue,
  * so it (and the calls that make it) must not run while another thread submits. No

@@ -441,6 +441,51 @@ def test_coalesced_host_batches_equal_separate_launches(w2vmod, storage, kernel)
             assert np.corrcoef(dg[m], dr[m])[0, 1] > 0.99
 
 
+def test_queued_host_batch_runs_before_device_step(orc, w2vmod):
+    """A host batch still queued on a stream is launched before device-sampling work that
+    is later submitted on that stream: one wave (order fixed), FP32, so submitting both
+    without a sync in between gives the same model bit for bit as with one, and the
+    opposite order gives another model."""
+    V, d, W, K = 200, 64, 3, 4
+    rng = np.random.default_rng(5)
+    syn0, syn1 = model(V, d, rng)
+    lens = rng.integers(5, 30, 40)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tokens = rng.integers(1, 60, off[-1]).astype(np.int32)
+    cn = np.bincount(tokens, minlength=V).astype(np.uint64) + 1
+    T = int(off[-1])
+    hl = [9, 14, 6]
+    hoff = np.concatenate([[0], np.cumsum(hl)]).astype(np.int64)
+    htok = rng.integers(1, 60, hoff[-1]).astype(np.int32)
+    hneg = rng.integers(1, V, hoff[-1] * K).astype(np.int32)
+    halpha = np.full(3, 0.05, np.float32)
+
+    def run(order):
+        tr = w2vmod.Trainer(V, d, W, K, sigmoid="exact", mode="hogwild", seed=9)
+        tr.set_launch_shape(1, 1)
+        tr.set_hot_rows(0)
+        tr.upload(syn0, syn1)
+        tr.tables_upload(orc.unigram_table(cn, 5003))  # no subsampling
+        tr.set_schedule(0.05, 1, T)
+        tr.corpus_upload(tokens, off)
+        if order == "device first":
+            tr.train_device(0, 0, len(lens))
+            tr.sync()
+        tr.submit_csr(htok, hoff, hneg, halpha)
+        if order == "synced":
+            tr.sync()
+        if order != "device first":
+            tr.train_device(0, 0, len(lens))
+        tr.sync()
+        out = tr.download()
+        tr.close()
+        return out
+
+    a, b, c = run("synced"), run("queued"), run("device first")
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[0], c[0])
+
+
 def test_coalesce_rejects_bad_limits(w2vmod):
     tr = w2vmod.Trainer(100, 16, 2, 1)
     for args in ((-1, 0), (10, -5), (1 << 31, 0)):
