@@ -327,6 +327,104 @@ def run_combined(w2v, steps, warmup):
     return results
 
 
+# The CPU reference configuration: V=71,290, 17M tokens in fixed 1000-token sentences,
+# d=128 W=5 K=5, sample 1e-4, 1 epoch. Small enough for the CPU trainer to run all of it.
+WL_SMALL = dict(WL, V=71_290, d=128, n_sent=17_000, median=0, len_max=1000, heldout_sent=2_000)
+
+
+def to_reference_stream(tok, off, ns):
+    """Reference corpus layout (improved_driver_functions.cpp:244-281 with -cache-eos): each
+    line followed by </s>, sentence starts into one stream ending in 0."""
+    lens = np.diff(off[:ns + 1])
+    starts = off[:ns] + np.arange(ns)
+    stream = np.zeros(off[ns] + ns + 1, np.int32)
+    for s in range(ns):
+        stream[starts[s]:starts[s] + lens[s]] = tok[off[s]:off[s + 1]]
+    return stream, starts.astype(np.int64), int(off[ns])
+
+
+def run_small(w2v, warmup):
+    """Both GPU kernels and the CPU trainer on the whole small corpus, one epoch each from
+    the same initial model. GPU: `warmup` untimed epochs on a scratch model, then the model
+    is re-initialised and one epoch (one launch) is timed. All three models are scored by
+    the same held-out evaluator (the library's loss kernel, same keyed negatives)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+
+    wl = WL_SMALL
+    V, d, W, K = wl["V"], wl["d"], wl["W"], wl["K"]
+    cfg = w2v.synth_config(wl["clusters"], wl["median"], wl["len_max"], wl["zipf"], wl["stay"],
+                           wl["in_cluster"], wl["sigma"], wl["seed"])
+    hcfg = w2v.synth_config(wl["clusters"], wl["median"], wl["len_max"], wl["zipf"], wl["stay"],
+                            wl["in_cluster"], wl["sigma"], wl["heldout_seed"])
+    labels = w2v.synth_cluster_of(wl["seed"], wl["clusters"], np.arange(V))
+    probe = np.arange(2, 2002)
+    host, res = {}, {}
+    for kernel, storage in variants(w2v):
+        tr = w2v.Trainer(V, d, W, K, msl=wl["len_max"], storage=storage, kernel=kernel,
+                         sigmoid="table", mode="hogwild", seed=wl["seed"])
+        if "heldout" not in host:
+            tr.synth_corpus(hcfg, wl["heldout_sent"])
+            host["heldout"] = tr.corpus_download(0, wl["heldout_sent"])
+        tr.synth_corpus(cfg, wl["n_sent"])
+        nt, ns = tr.corpus_info()
+        if "counts" not in host:
+            cn = tr.corpus_counts()
+            cn[0] = ns
+            host.update(counts=cn, unigram=w2v.unigram_table(cn, wl["unigram"]),
+                        subs=w2v.subsample_probs(cn, wl["sample"], nt), corpus=tr.corpus_download(0, ns),
+                        nt=nt, ns=ns)
+        tr.tables_upload(host["unigram"], host["subs"])
+        hot = tr.auto_hot_rows(host["counts"], wl["sample"], nt)
+        tr.set_schedule(wl["alpha"], 1, nt)
+        for _ in range(warmup):
+            tr.init_model()
+            tr.train_device(0, 0, ns)
+        tr.init_model()
+        tr.sync()
+        ms0, n0 = tr.kernel_time(0)
+        st0 = tr.stats()
+        t0 = time.perf_counter()
+        tr.train_device(0, 0, ns)
+        tr.sync()
+        secs = time.perf_counter() - t0
+        ms1, n1 = tr.kernel_time(0)
+        st1 = tr.stats()
+        words = st1["words_read"] - st0["words_read"]
+        kept = st1["words_trained"] - st0["words_trained"]
+        loss, pairs = tr.heldout_loss(*host["heldout"])
+        s0, _ = tr.download()
+        tr.close()
+        rec = nn_recovery(s0, labels, probe)
+        avg_ms = (ms1 - ms0) / max(n1 - n0, 1)
+        bpw = bytes_per_word(d, K, 2 if storage == "bf16" else 4)
+        res[kernel] = dict(kernel=kernel, storage=storage, words_per_sec=words / secs, seconds=secs,
+                           words=int(words), kept_words=int(kept), kernel_ms=avg_ms,
+                           achieved_gbs=kept * bpw / (avg_ms / 1e3) / 1e9, hot_rows=int(hot),
+                           heldout_loss=loss, heldout_pairs=int(pairs), nn_recovery=rec)
+        log(f"[small {kernel}/{storage}] {words / secs / 1e6:.1f} M words/s ({avg_ms:.1f} ms), "
+            f"held-out loss {loss:.4f}, NN recovery {rec:.4f}, {hot} hot rows")
+    tok, off = host["corpus"]
+    stream, starts, nw = to_reference_stream(tok, off, host["ns"])
+    threads = min(16, len(os.sched_getaffinity(0)))
+    c0 = orc.init_syn0(V, d)
+    c1 = np.zeros((V, d), np.float32)
+    wps, cw, csecs = orc.train_hogwild(stream, starts, nw, host["unigram"], host["subs"], c0, c1, d, W, K,
+                                       wl["len_max"], 1, threads, wl["alpha"], wl["sample"])
+    ev = w2v.Trainer(V, d, W, K, msl=wl["len_max"], sigmoid="table", mode="hogwild", seed=wl["seed"])
+    ev.tables_upload(host["unigram"], host["subs"])
+    ev.upload(c0, c1)
+    closs, _ = ev.heldout_loss(*host["heldout"])
+    ev.close()
+    crec = nn_recovery(c0, labels, probe)
+    log(f"[small cpu] {wps / 1e6:.2f} M words/s on {threads} threads ({cw} words, {csecs:.1f}s), "
+        f"held-out loss {closs:.4f}, NN recovery {crec:.4f}")
+    cpu = dict(value=wps, unit="words/s", cores=threads, kind="port",
+               sample=f"the whole corpus ({cw} tokens), 1 epoch, FP32 Hogwild, table sigmoid",
+               heldout_loss=closs, nn_recovery=crec)
+    return res, cpu, host
+
+
 def cpu_baseline(host, target_s=15.0, wl=WL):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
@@ -398,7 +496,8 @@ def main():
     ap.add_argument("--streamed", action="store_true",
                     help="also time the MFMA path with the corpus streamed from host memory")
     ap.add_argument("--workload", default="text",
-                    help="text (the headline workload) | d128 (V=555k, d=128, 1B tokens) | walks | combined")
+                    help="text (the headline workload) | d128 (V=555k, d=128, 1B tokens) | "
+                         "small (V=71,290, 17M tokens, with the CPU trainer on all of it) | walks | combined")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to scale HBM byte counters")
     args = ap.parse_args()
@@ -423,6 +522,29 @@ def main():
                                    "30% truncated to U[5,80], d=128 window 10 negative 5 sample 1e-4, 1 epoch",
                        "best": best[0], "parallelism": "single-gpu"},
             "variants": res}), flush=True)
+        return
+    if args.workload == "small":
+        res, cpu, host = run_small(w2v, max(args.warmup, 1))
+        best = max(res.values(), key=lambda r: r["words_per_sec"])
+        line = {
+            "metric": "trained words/sec, V=71,290 d=128 w=5 neg=5, 17M tokens in 1000-token sentences, "
+                      "against the CPU trainer on the whole corpus",
+            "value": best["words_per_sec"], "unit": "words/s", "n_gpus": 1, "steps": 1,
+            "warmup": max(args.warmup, 1), "ms_per_step": best["seconds"] * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16" if best["storage"] == "bf16" else "fp32",
+            "data": "synthetic",
+            "config": {"workload": "cluster-Markov Zipf corpus, V=71,290, 64 clusters, 17,000 sentences of "
+                                   "1000 tokens, d=128 window 5 negative 5 sample 1e-4, 1 epoch (one step)",
+                       "tokens": int(host["nt"]), "kernel": best["kernel"], "parallelism": "single-gpu"},
+            "roofline": {"bound": "hbm", "achieved": best["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": best["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None},
+            "cpu_baseline": cpu, "variants": res,
+            "gpu_vs_cpu": {k: v["words_per_sec"] / cpu["value"] for k, v in res.items()},
+            "heldout_loss_rel_diff_vs_cpu": {k: (v["heldout_loss"] - cpu["heldout_loss"]) / cpu["heldout_loss"]
+                                             for k, v in res.items()},
+            "nn_recovery_diff_vs_cpu": {k: v["nn_recovery"] - cpu["nn_recovery"] for k, v in res.items()},
+        }
+        print(json.dumps(line), flush=True)
         return
     if args.workload == "combined":
         res = run_combined(w2v, args.steps, args.warmup)
