@@ -543,7 +543,8 @@ bool Corpus::read_bin(const std::string& path, int msl, bool cache_eos, int64_t 
     *err = "not a binary corpus cache";
     return false;
   }
-  if (f.size != sizeof(h) + h.n_tokens * 4 + h.n_sent * 8) {
+  if (h.n_tokens > f.size / 4 || h.n_sent > f.size / 8 ||  // (no overflow in the sum below)
+      f.size != sizeof(h) + h.n_tokens * 4 + h.n_sent * 8) {
     *err = "size does not match its header";
     return false;
   }
