@@ -183,6 +183,25 @@ def test_cli_reference_sweep_streams(tmp_path):
     assert len(txt) == int(txt[0].split()[0]) + 1
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("sampling", ["0", "1"])
+def test_cli_trains_from_binary_corpus_cache(tmp_path, sampling):
+    """Training from -read-corpus-bin reads the same corpus as training from the text:
+    the same word count (the reference's 'Overall Words'), and a finite export."""
+    words_text(tmp_path, 2000)
+    common = ["-size", "32", "-window", "3", "-negative", "4", "-epochs", "2", "-threads", "2",
+              "-cache-eos", "-min-count", "1", "-sample", "1e-3", "-device-sampling", sampling]
+    r1 = run(["-train", "big.txt", "-save-vocab", "v.txt", "-save-corpus-bin", "c.bin", "-output", "a.txt"]
+             + common, tmp_path, check=0)
+    r2 = run(["-read-vocab", "v.txt", "-read-corpus-bin", "c.bin", "-output", "b.txt"] + common, tmp_path,
+             check=0)
+    assert parse_contract(r1.stdout)[1] == parse_contract(r2.stdout)[1] > 0
+    txt = (tmp_path / "b.txt").read_text().splitlines()
+    assert len(txt) == int(txt[0].split()[0]) + 1
+    vals = np.array([[float(x) for x in line.split()[1:]] for line in txt[1:]])
+    assert np.isfinite(vals).all() and np.abs(vals).max() > 0
+
+
 def test_binary_corpus_cache_round_trip(tmp_path):
     """-save-corpus-bin then -read-corpus-bin gives the same corpus (compared through the
     reference's text cache); other -msl / -cache-eos values and a damaged file are refused."""
