@@ -74,7 +74,7 @@ def variants(w2v):
     return out
 
 
-def run_variant(w2v, kernel, storage, steps, warmup, host, wl=WL):
+def run_variant(w2v, kernel, storage, steps, warmup, host, wl=WL, piece=0):
     V, d, W, K = wl["V"], wl["d"], wl["W"], wl["K"]
     tr = w2v.Trainer(V, d, W, K, msl=wl["len_max"], storage=storage, kernel=kernel,
                      sigmoid="table", mode="hogwild", seed=wl["seed"])
@@ -93,6 +93,7 @@ def run_variant(w2v, kernel, storage, steps, warmup, host, wl=WL):
         host["tokens"], host["nt"], host["ns"] = None, nt, ns
     tr.tables_upload(host["unigram"], host["subs"])
     hot = tr.auto_hot_rows(host["counts"], wl["sample"], nt)
+    tr.set_piece_length(piece)
     epochs = max(wl.get("epochs", 1), math.ceil((steps + warmup) / CHUNKS))
     tr.set_schedule(wl["alpha"], epochs, nt)
     tr.init_model()
@@ -266,7 +267,7 @@ COMBINED = dict(V=4_000_000, d=300, W=10, K=15, sample=1e-4, alpha=0.025, tokens
                 seed=1, heldout_seed=2, heldout_sent=5_000, parts=100)
 
 
-def run_combined(w2v, steps, warmup):
+def run_combined(w2v, steps, warmup, piece=0):
     """The survey's combined configuration at full size: vocabulary 4M (Zipf 1.05, 256
     clusters), 5B tokens in sentences of log-normal length (median 22, max 4000), d=300,
     window 10, 15 negatives, model resident in HBM (bf16 rows for the MFMA kernel, FP32 for
@@ -293,6 +294,7 @@ def run_combined(w2v, steps, warmup):
             host["subs"] = w2v.subsample_probs(cn, c["sample"], nt)
         tr.tables_upload(host["uni"], host["subs"])
         hot = tr.auto_hot_rows(host["cn"], c["sample"], nt)
+        tr.set_piece_length(piece)
         tr.set_schedule(c["alpha"], 1, nt)
         tr.init_model()
         bounds = [round(i * ns / c["parts"]) for i in range(c["parts"] + 1)]
@@ -343,7 +345,7 @@ def to_reference_stream(tok, off, ns):
     return stream, starts.astype(np.int64), int(off[ns])
 
 
-def run_small(w2v, warmup):
+def run_small(w2v, warmup, piece=64):
     """Both GPU kernels and the CPU trainer on the whole small corpus, one epoch each from
     the same initial model. GPU: `warmup` untimed epochs on a scratch model, then the model
     is re-initialised and one epoch (one launch) is timed. All three models are scored by
@@ -376,6 +378,7 @@ def run_small(w2v, warmup):
                         nt=nt, ns=ns)
         tr.tables_upload(host["unigram"], host["subs"])
         hot = tr.auto_hot_rows(host["counts"], wl["sample"], nt)
+        tr.set_piece_length(piece)
         tr.set_schedule(wl["alpha"], 1, nt)
         for _ in range(warmup):
             tr.init_model()
@@ -401,7 +404,7 @@ def run_small(w2v, warmup):
         res[kernel] = dict(kernel=kernel, storage=storage, words_per_sec=words / secs, seconds=secs,
                            words=int(words), kept_words=int(kept), kernel_ms=avg_ms,
                            achieved_gbs=kept * bpw / (avg_ms / 1e3) / 1e9, hot_rows=int(hot),
-                           heldout_loss=loss, heldout_pairs=int(pairs), nn_recovery=rec)
+                           heldout_loss=loss, heldout_pairs=int(pairs), nn_recovery=rec, piece_length=piece)
         log(f"[small {kernel}/{storage}] {words / secs / 1e6:.1f} M words/s ({avg_ms:.1f} ms), "
             f"held-out loss {loss:.4f}, NN recovery {rec:.4f}, {hot} hot rows")
     tok, off = host["corpus"]
@@ -498,6 +501,9 @@ def main():
     ap.add_argument("--workload", default="text",
                     help="text (the headline workload) | d128 (V=555k, d=128, 1B tokens) | "
                          "small (V=71,290, 17M tokens, with the CPU trainer on all of it) | walks | combined")
+    ap.add_argument("--piece-length", type=int, default=None,
+                    help="train sentences as pieces of at most this many centres (w2v_set_piece_length); "
+                         "default 64 for the small workload, whole sentences otherwise")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to scale HBM byte counters")
     args = ap.parse_args()
@@ -524,7 +530,8 @@ def main():
             "variants": res}), flush=True)
         return
     if args.workload == "small":
-        res, cpu, host = run_small(w2v, max(args.warmup, 1))
+        piece = 64 if args.piece_length is None else args.piece_length
+        res, cpu, host = run_small(w2v, max(args.warmup, 1), piece)
         best = max(res.values(), key=lambda r: r["words_per_sec"])
         line = {
             "metric": "trained words/sec, V=71,290 d=128 w=5 neg=5, 17M tokens in 1000-token sentences, "
@@ -535,7 +542,8 @@ def main():
             "data": "synthetic",
             "config": {"workload": "cluster-Markov Zipf corpus, V=71,290, 64 clusters, 17,000 sentences of "
                                    "1000 tokens, d=128 window 5 negative 5 sample 1e-4, 1 epoch (one step)",
-                       "tokens": int(host["nt"]), "kernel": best["kernel"], "parallelism": "single-gpu"},
+                       "tokens": int(host["nt"]), "kernel": best["kernel"], "piece_length": piece,
+                       "parallelism": "single-gpu"},
             "roofline": {"bound": "hbm", "achieved": best["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": best["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None},
             "cpu_baseline": cpu, "variants": res,
@@ -547,7 +555,7 @@ def main():
         print(json.dumps(line), flush=True)
         return
     if args.workload == "combined":
-        res = run_combined(w2v, args.steps, args.warmup)
+        res = run_combined(w2v, args.steps, args.warmup, args.piece_length or 0)
         best = max(res.items(), key=lambda kv: kv[1]["words_per_sec"])
         print(json.dumps({
             "metric": "trained words/sec, 4M vocabulary, d=300 w=10 neg=15, sentences up to 4000 tokens",
@@ -567,7 +575,8 @@ def main():
     wl = WL_D128 if args.workload == "d128" else WL
     results = {}
     for kernel, storage in vs:
-        results[kernel] = run_variant(w2v, kernel, storage, args.steps, args.warmup, host, wl)
+        results[kernel] = run_variant(w2v, kernel, storage, args.steps, args.warmup, host, wl,
+                                      args.piece_length or 0)
     streamed = run_streamed(w2v, args.steps, args.warmup, host) if args.streamed and wl is WL else None
     cpu = None if args.no_cpu else cpu_baseline(host, wl=wl)
     best = max(results.values(), key=lambda r: r["words_per_sec"])
@@ -593,7 +602,8 @@ def main():
                          "(median 22, max 1000), d=300 window 5 negative 5 sample 1e-4, 1 epoch"),
             "vocab": wl["V"], "tokens": int(host["nt"]), "sentences": int(host["ns"]),
             "layer_size": wl["d"], "window": wl["W"], "negative": wl["K"],
-            "kernel": best["kernel"], "storage": best["storage"], "parallelism": "single-gpu",
+            "kernel": best["kernel"], "storage": best["storage"], "piece_length": args.piece_length or 0,
+            "parallelism": "single-gpu",
         },
         "roofline": {
             "bound": "hbm",

@@ -31,6 +31,9 @@ hipError_t launch_walk_tokens(int64_t n_walks, int64_t n_nodes, uint64_t ms, con
                               const int32_t* adj, const int64_t* soff, int32_t base, int32_t* out,
                               hipStream_t st);
 size_t compact_temp_bytes(int64_t n);
+size_t piece_temp_bytes(int64_t ns);
+hipError_t launch_pieces(const int64_t* kept_off, int64_t ns, int32_t piece_len, int32_t* cnt, int32_t* start,
+                         int64_t* pieces, int64_t* n_pieces, void* temp, size_t temp_bytes, hipStream_t st);
 hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t sb, int64_t se,
                           const float* subs_p, uint64_t mixed_seed, int32_t epoch,
                           int32_t* flags, int32_t* pos, int32_t* kept, int64_t* kept_off,
@@ -71,6 +74,8 @@ struct StreamState {
   DevBuf d_tok, d_neg, d_len, d_alpha, d_off;
   // device-corpus step buffers
   DevBuf flags, pos, kept, kept_off, temp;
+  // sentence pieces of a step (w2v_set_piece_length): counts, their scan, the list, its size
+  DevBuf pcnt, pstart, pieces, npieces;
   // graph of a captured epoch (w2v_train_epoch_graph) and the step bounds it was built for
   hipGraphExec_t gexec = nullptr;
   std::vector<int64_t> gkey;
@@ -97,6 +102,7 @@ struct w2v_ctx {
   int max_blocks[2] = {0, 0};  // device-filling grid at cfg_wpb
   int user_wpb = 0, user_blocks = 0;  // w2v_set_launch_shape override
   int32_t hot_rows = W2V_DEFAULT_HOT_ROWS;  // w2v_set_hot_rows
+  int32_t piece_len = 0;                     // w2v_set_piece_length
   int64_t coalesce_sent = W2V_DEFAULT_COALESCE_SENTENCES, coalesce_tok = W2V_DEFAULT_COALESCE_TOKENS;
   int occ_bpc[5] = {0, 0, 0, 0, 0};   // Hogwild: resident workgroups per CU at 1..4 waves each
   std::string arch;                   // device name (gcnArchName), the tuning cache key
@@ -444,7 +450,7 @@ int w2v_destroy(w2v_ctx* c) {
     for (HostBuf* h : {&s.h_tok, &s.h_neg, &s.h_len, &s.h_alpha, &s.h_off})
       if (h->p) hipHostFree(h->p);
     for (DevBuf* d : {&s.d_tok, &s.d_neg, &s.d_len, &s.d_alpha, &s.d_off, &s.flags, &s.pos, &s.kept,
-                      &s.kept_off, &s.temp})
+                      &s.kept_off, &s.temp, &s.pcnt, &s.pstart, &s.pieces, &s.npieces})
       if (d->p) hipFree(d->p);
     if (s.queue) hipFree(s.queue);
     if (s.gexec) hipGraphExecDestroy(s.gexec);
@@ -892,8 +898,35 @@ int w2v_corpus_upload(w2v_ctx* c, const int32_t* tokens, int64_t n_tokens, const
 }
 
 // Buffers of one device-path step of up to n corpus tokens in up to ns sentences.
+// Piece buffers for a step of n tokens in ns sentences (no-op when pieces are off).
+static int ensure_piece_buffers(w2v_ctx* c, StreamState& s, int64_t n, int64_t ns) {
+  if (c->piece_len <= 0) return W2V_OK;
+  const int64_t most = ns + n / c->piece_len + 1;
+  if (most >= ((int64_t)1 << 31) - 1) return fail(c, W2V_ERR_ARG, "too many sentence pieces in one step");
+  int rc;
+  if ((rc = ensure_dev(c, s.pcnt, (size_t)(ns + 1) * 4)) || (rc = ensure_dev(c, s.pstart, (size_t)(ns + 1) * 4)) ||
+      (rc = ensure_dev(c, s.pieces, (size_t)most * 8)) || (rc = ensure_dev(c, s.npieces, 8)) ||
+      (rc = ensure_dev(c, s.temp, piece_temp_bytes(ns))))
+    return rc;
+  return W2V_OK;
+}
+
+// Splits the step's sentences (kept offsets in s.kept_off) into work items of at most
+// piece_len centres; the update kernel then takes the piece list instead of sentences.
+static int add_pieces(w2v_ctx* c, StreamState& s, UpdateArgs& a, int64_t ns) {
+  if (c->piece_len <= 0) return W2V_OK;
+  HIP_TRY(c, launch_pieces((const int64_t*)s.kept_off.p, ns, c->piece_len, (int32_t*)s.pcnt.p,
+                           (int32_t*)s.pstart.p, (int64_t*)s.pieces.p, (int64_t*)s.npieces.p, s.temp.p,
+                           s.temp.cap, s.st));
+  a.pieces = (const int64_t*)s.pieces.p;
+  a.n_pieces = (const int64_t*)s.npieces.p;
+  a.piece_len = c->piece_len;
+  return W2V_OK;
+}
+
 static int ensure_step_buffers(w2v_ctx* c, StreamState& s, int64_t n, int64_t ns) {
   int rc;
+  if ((rc = ensure_piece_buffers(c, s, n, ns))) return rc;
   const size_t tb_bytes = compact_temp_bytes(n);
   if ((rc = ensure_dev(c, s.flags, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.pos, (size_t)(n + 1) * 4)) ||
       (rc = ensure_dev(c, s.kept, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.kept_off, (size_t)(ns + 1) * 8)) ||
@@ -924,6 +957,8 @@ static int enqueue_device_step(w2v_ctx* c, StreamState& s, int32_t epoch, int64_
   a.total_words = (uint64_t)c->epochs * c->train_words + 1;
   a.epoch_words = (uint64_t)epoch * c->train_words;
   a.n_sent = se - sb;
+  int rc = add_pieces(c, s, a, se - sb);
+  if (rc) return rc;
   return launch_update(c, s, a, timed);
 }
 
@@ -1003,6 +1038,7 @@ int w2v_train_epoch_graph(w2v_ctx* c, int32_t stream, int32_t epoch, const int64
   // the epoch is a kernel argument: refresh the cached executable for the same steps,
   // instantiate a new one otherwise
   std::vector<int64_t> key(bounds, bounds + n_steps + 1);
+  key.push_back(c->piece_len);
   if (s.gexec && s.gkey == key) {
     hipGraphNode_t bad = nullptr;
     hipGraphExecUpdateResult res;
@@ -1059,7 +1095,8 @@ int w2v_train_chunk(w2v_ctx* c, int32_t stream, int32_t epoch, const int32_t* to
   HIP_TRY(c, hipStreamSynchronize(s.st));  // this stream's staging is free again
   const size_t bt = (size_t)n * 4, bo = (size_t)(ns + 1) * 8;
   const size_t tb_bytes = compact_temp_bytes(n);
-  if ((rc = ensure_host(c, s.h_tok, bt)) || (rc = ensure_host(c, s.h_off, bo)) ||
+  if ((rc = ensure_piece_buffers(c, s, n, ns)) || (rc = ensure_host(c, s.h_tok, bt)) ||
+      (rc = ensure_host(c, s.h_off, bo)) ||
       (rc = ensure_dev(c, s.d_tok, bt)) || (rc = ensure_dev(c, s.d_off, bo)) ||
       (rc = ensure_dev(c, s.flags, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.pos, (size_t)(n + 1) * 4)) ||
       (rc = ensure_dev(c, s.kept, (size_t)(n + 1) * 4)) || (rc = ensure_dev(c, s.kept_off, bo)) ||
@@ -1086,6 +1123,7 @@ int w2v_train_chunk(w2v_ctx* c, int32_t stream, int32_t epoch, const int32_t* to
   a.total_words = (uint64_t)c->epochs * c->train_words + 1;
   a.epoch_words = (uint64_t)epoch * c->train_words;
   a.n_sent = ns;
+  if ((rc = add_pieces(c, s, a, ns))) return rc;
   c->words_read += (uint64_t)n;
   return launch_update(c, s, a);
 }
@@ -1215,6 +1253,15 @@ int w2v_set_launch_shape(w2v_ctx* c, int32_t wpb, int32_t blocks) {
   if (!c || wpb < 0 || wpb > 4 || blocks < 0) return W2V_ERR_ARG;
   c->user_wpb = wpb;
   c->user_blocks = blocks;
+  return W2V_OK;
+}
+
+int w2v_set_piece_length(w2v_ctx* c, int32_t n) {
+  if (!c) return W2V_ERR_ARG;
+  if (n < 0) return fail(c, W2V_ERR_ARG, "piece length must be >= 0 (0: whole sentences)");
+  int rc = w2v_sync_all(c);  // launches in flight read the current piece lists
+  if (rc) return rc;
+  c->piece_len = n;
   return W2V_OK;
 }
 

@@ -83,6 +83,47 @@ hipError_t launch_compact(const int32_t* corpus, const int64_t* raw_off, int64_t
   return hipGetLastError();
 }
 
+// Sentence pieces of one step: cnt[s] = ceil(kept length / piece_len) (0 for an empty
+// sentence, cnt[ns] = 0), an exclusive scan into start[], then one entry per piece.
+__global__ void piece_counts_kernel(const int64_t* __restrict__ kept_off, int64_t ns, int32_t piece_len,
+                                    int32_t* __restrict__ cnt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  cnt[s] = s == ns ? 0 : (int32_t)((kept_off[s + 1] - kept_off[s] + piece_len - 1) / piece_len);
+}
+
+__global__ void piece_list_kernel(int64_t ns, int32_t piece_len, const int32_t* __restrict__ cnt,
+                                  const int32_t* __restrict__ start, int64_t* __restrict__ pieces,
+                                  int64_t* __restrict__ n_pieces) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  if (s == ns) {
+    *n_pieces = (int64_t)start[ns];
+    return;
+  }
+  for (int32_t k = 0; k < cnt[s]; k++) pieces[(int64_t)start[s] + k] = (s << 32) | ((int64_t)k * piece_len);
+}
+
+size_t piece_temp_bytes(int64_t ns) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr,
+                                         (int)(ns + 1));
+  return bytes;
+}
+
+hipError_t launch_pieces(const int64_t* kept_off, int64_t ns, int32_t piece_len, int32_t* cnt, int32_t* start,
+                         int64_t* pieces, int64_t* n_pieces, void* temp, size_t temp_bytes, hipStream_t st) {
+  const int B = 256;
+  const unsigned g = (unsigned)((ns + 1 + B - 1) / B);
+  hipLaunchKernelGGL(piece_counts_kernel, dim3(g), dim3(B), 0, st, kept_off, ns, piece_len, cnt);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, cnt, start, (int)(ns + 1), st);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(piece_list_kernel, dim3(g), dim3(B), 0, st, ns, piece_len, cnt, start, pieces, n_pieces);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------
 // Held-out loss: one wave per sentence, dots in FP32 lanes, per-pair loss and the
 // accumulation in FP64.

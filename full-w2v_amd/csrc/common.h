@@ -136,6 +136,12 @@ struct UpdateArgs {
   int32_t hot_rows;
   const float* exp_t;
   const float* nexp_t;
+  // sentence pieces (device-sampling runs; nullptr = one work item per sentence): item i
+  // trains centres [c0, c0 + piece_len) of sentence s, pieces[i] = s << 32 | c0, with its
+  // windows' context rows taken from the whole sentence; *n_pieces items in all
+  const int64_t* pieces;
+  const int64_t* n_pieces;
+  int32_t piece_len;
   // work queue and counters
   unsigned int* queue;
   unsigned long long* counters;  // [0] words trained, [1] sentences

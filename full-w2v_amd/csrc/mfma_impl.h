@@ -142,12 +142,20 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
   typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
   u32x16_t u0v[NP];  // load-time ring pairs when held in registers (element = ring slot)
   u32x8_t u0w[MT == 2 ? NP : 1];  // slots 16..23 of a two-tile ring
+  const int64_t n_items = a.pieces ? *a.n_pieces : a.n_sent;
 
   for (;;) {
     unsigned s32 = 0;
     if (lane == 0) s32 = atomicAdd(a.queue, 1u);
-    const int64_t s = (int64_t)(unsigned)uniform_i((int)s32);
-    if (s >= a.n_sent) break;
+    const int64_t item = (int64_t)(unsigned)uniform_i((int)s32);
+    if (item >= n_items) break;
+    int64_t s = item;
+    int c0 = 0;  // first centre of this work item
+    if (a.pieces) {
+      const int64_t pc = a.pieces[item];
+      s = pc >> 32;
+      c0 = (int)(pc & 0xFFFFFFFF);
+    }
     int64_t base;
     int L;
     if (a.offsets) {
@@ -157,7 +165,10 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       base = s * (int64_t)a.msl;
       L = a.length[s];
     }
-    if (L <= 0) continue;
+    if (L <= c0) continue;
+    // centres [c0, c1); context positions [plo, phi) (the windows of those centres)
+    const int c1 = a.pieces && c0 + a.piece_len < L ? c0 + a.piece_len : L;
+    const int plo = c0 - W > 0 ? c0 - W : 0, phi = c1 + W < L ? c1 + W : L;
     float alpha;
     int64_t raw = 0;
     if (a.alpha) {
@@ -168,16 +179,16 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       alpha = a.alpha0 * (1.0f - (float)wca / (float)a.total_words);
       if (alpha < a.alpha0 * 0.0001f) alpha = a.alpha0 * 0.0001f;
     }
-    words += (unsigned long long)L;
-    sents += 1;
-    uint32_t rng = (uint32_t)key2(a.mixed_seed, 0x5352ull ^ ((uint64_t)a.epoch << 40), (uint64_t)(s * 64 + lane)) | 1u;
+    words += (unsigned long long)(c1 - c0);
+    sents += c0 == 0 ? 1 : 0;
+    uint32_t rng = (uint32_t)key2(a.mixed_seed, 0x5352ull ^ ((uint64_t)a.epoch << 40), (uint64_t)(item * 64 + lane)) | 1u;
 
     // Token ids and negatives are staged 64 lanes at a time, with the following batch
     // loaded one batch ahead: the registers read in the centre loop are only ever
     // written by moves, so no wait on outstanding memory operations is needed there.
-    int chunk0 = 0;
-    int tokbuf = (lane < L) ? a.tokens[base + lane] : 0;
-    int toknext = (64 + lane < L) ? a.tokens[base + 64 + lane] : 0;
+    int chunk0 = plo;
+    int tokbuf = (plo + lane < L) ? a.tokens[base + plo + lane] : 0;
+    int toknext = (plo + 64 + lane < L) ? a.tokens[base + plo + 64 + lane] : 0;
     int ring_tok = 0;
     auto neg_batch = [&](int nw) -> int {  // negatives of centres nw .. nw+cpb-1
       const int cw = nw + lane / K, kk = lane % K;
@@ -192,9 +203,9 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       }
       return v;
     };
-    int nw0 = 0;
-    int negbuf = K > 0 ? neg_batch(0) : 0;
-    int negnext = K > 0 ? neg_batch(cpb) : 0;
+    int nw0 = c0;
+    int negbuf = K > 0 ? neg_batch(c0) : 0;
+    int negnext = K > 0 ? neg_batch(c0 + cpb) : 0;
 
     // load-time bf16 pairs of ring slot `slot` (wave-uniform): registers or LDS. The
     // register copy is a 16-element vector per pair indexed by the uniform slot, which
@@ -291,11 +302,13 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
     };
     auto negative = [&](int w, int k) -> int { return readlane_i(negbuf, (w - nw0) * K + k); };
 
-    const int npre = L < W + 1 ? L : W + 1;
-    for (int p = 0; p < npre; p++) {
-      const int tok = readlane_i(tokbuf, p);
-      ring_tok = (lane == p) ? tok : ring_tok;
-      load_row(p, tok);
+    // preload positions plo..c0+W (0..W for a whole sentence); ring slot = position mod R
+    const int pend = c0 + W + 1 < phi ? c0 + W + 1 : phi;
+    for (int p = plo; p < pend; p++) {
+      const int tok = readlane_i(tokbuf, p - chunk0);
+      const int sl = p % R;
+      ring_tok = (lane == sl) ? tok : ring_tok;
+      load_row(sl, tok);
     }
     // Hogwild mode loads the next centre's target rows and the next entering context
     // row one centre ahead (packed bf16 pairs); deterministic mode loads in order.
@@ -325,17 +338,17 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       }
     };
     if (pipe) {
-      targets_of(0, 0, ntk);
+      targets_of(c0, c0 % R, ntk);
 #pragma unroll
       for (int k = 0; k < NTB; k++)
         if (k < NT) fetch_pairs(a.bsyn1, ntk[k], nT[k]);
     }
 
-    int cs = 0;
-    for (int w = 0; w < L; w++) {
+    int cs = c0 % R;
+    for (int w = c0; w < c1; w++) {
       const int enter_pos = w + W + 1;
       int enter_tok = 0;
-      if (enter_pos < L) {
+      if (enter_pos < phi) {
         enter_tok = entering_token(enter_pos);
         if (pipe) fetch_pairs(a.bsyn0, enter_tok, nE);
       }
@@ -350,7 +363,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
 #pragma unroll
           for (int e = 0; e < NP; e++) cT[k][e] = nT[k][e];
         }
-        if (w + 1 < L) {  // next centre's targets
+        if (w + 1 < c1) {  // next centre's targets
           next_centre(w + 1);
           targets_of(w + 1, cs + 1 == R ? 0 : cs + 1, ntk);
 #pragma unroll
@@ -434,7 +447,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
         int rel = lane - cs;
         if (rel < 0) rel += R;
         if (rel > W) rel -= R;
-        const bool hot = lane < R && rel != -W && w + rel >= 0 && w + rel < L && ring_tok < a.hot_rows;
+        const bool hot = lane < R && rel != -W && w + rel >= plo && w + rel < phi && ring_tok < a.hot_rows;
         const uint64_t hm = __ballot(hot);
         if (hm) {
           hot_any = true;
@@ -531,7 +544,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       uint32_t lp[NP];
       int ltok = 0;
       if (pipe) {
-        if (leave >= 0) {
+        if (leave >= plo) {
           ltok = readlane_i(ring_tok, slot);
           const float* uf = Uf + slot * ly.PU;
           u0_get(slot, lp);
@@ -542,7 +555,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
             lv[e] = make_float2(u.x - bf_lo(lp[e]), u.y - bf_hi(lp[e]));
           }
         }
-        if (enter < L) {
+        if (enter < phi) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           float* uf = Uf + slot * ly.PU;
           u0_put(slot, nE);
@@ -606,7 +619,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
 
       // ---- ring advance: flush the leaving context, load the entering one
       if (pipe) {
-        if (leave >= 0) {
+        if (leave >= plo) {
 #pragma unroll
           for (int e = 0; e < NP; e++) {
             const int j = 2 * lane + 128 * e;
@@ -615,11 +628,11 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
           }
         }
       } else {
-        if (leave >= 0) {
+        if (leave >= plo) {
           flush_row(slot, readlane_i(ring_tok, slot));
           if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
         }
-        if (enter < L) {
+        if (enter < phi) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           load_row(slot, enter_tok);
         }
@@ -628,7 +641,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       __builtin_amdgcn_wave_barrier();
       cs = (cs + 1 == R) ? 0 : cs + 1;
     }
-    for (int p = (L - W > 0 ? L - W : 0); p < L; p++) {
+    for (int p = (c1 - W > plo ? c1 - W : plo); p < phi; p++) {
       const int slot = p % R;
       flush_row(slot, readlane_i(ring_tok, slot));
     }

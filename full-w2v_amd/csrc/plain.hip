@@ -65,12 +65,20 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
   float* Ds = U0s + (size_t)R * P;
   const int cpb = K > 0 ? 64 / K : 0;  // centres whose negatives fit in one VGPR
   unsigned long long words = 0, sents = 0;
+  const int64_t n_items = a.pieces ? *a.n_pieces : a.n_sent;
 
   for (;;) {
     unsigned s32 = 0;
     if (lane == 0) s32 = atomicAdd(a.queue, 1u);
-    const int64_t s = (int64_t)(unsigned)uniform_i((int)s32);
-    if (s >= a.n_sent) break;
+    const int64_t item = (int64_t)(unsigned)uniform_i((int)s32);
+    if (item >= n_items) break;
+    int64_t s = item;
+    int c0 = 0;  // first centre of this work item
+    if (a.pieces) {
+      const int64_t pc = a.pieces[item];
+      s = pc >> 32;
+      c0 = (int)(pc & 0xFFFFFFFF);
+    }
 
     int64_t base;
     int L;
@@ -81,7 +89,10 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       base = s * (int64_t)a.msl;
       L = a.length[s];
     }
-    if (L <= 0) continue;
+    if (L <= c0) continue;
+    // centres [c0, c1); context positions [plo, phi) (the windows of those centres)
+    const int c1 = a.pieces && c0 + a.piece_len < L ? c0 + a.piece_len : L;
+    const int plo = c0 - W > 0 ? c0 - W : 0, phi = c1 + W < L ? c1 + W : L;
     float alpha;
     int64_t raw = 0;
     if (a.alpha) {
@@ -93,15 +104,15 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       alpha = a.alpha0 * (1.0f - (float)wca / (float)a.total_words);
       if (alpha < a.alpha0 * 0.0001f) alpha = a.alpha0 * 0.0001f;
     }
-    words += (unsigned long long)L;
-    sents += 1;
+    words += (unsigned long long)(c1 - c0);
+    sents += c0 == 0 ? 1 : 0;
 
     // Token ids and negatives are staged 64 lanes at a time, with the following batch
     // loaded one batch ahead: the registers read in the centre loop are only ever
     // written by moves, so no wait on outstanding memory operations is needed there.
-    int chunk0 = 0;
-    int tokbuf = (lane < L) ? a.tokens[base + lane] : 0;
-    int toknext = (64 + lane < L) ? a.tokens[base + 64 + lane] : 0;
+    int chunk0 = plo;
+    int tokbuf = (plo + lane < L) ? a.tokens[base + plo + lane] : 0;
+    int toknext = (plo + 64 + lane < L) ? a.tokens[base + plo + 64 + lane] : 0;
     int ring_tok = 0;  // lane r holds the token id of ring slot r
     auto neg_batch = [&](int nw) -> int {  // negatives of centres nw .. nw+cpb-1
       const int cw = nw + lane / K, kk = lane % K;
@@ -116,9 +127,9 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       }
       return v;
     };
-    int nw0 = 0;
-    int negbuf = K > 0 ? neg_batch(0) : 0;
-    int negnext = K > 0 ? neg_batch(cpb) : 0;
+    int nw0 = c0;
+    int negbuf = K > 0 ? neg_batch(c0) : 0;
+    int negnext = K > 0 ? neg_batch(c0 + cpb) : 0;
 
     auto load_row = [&](int slot, int tok) {
       const float* src = a.syn0 + (size_t)tok * P;
@@ -181,12 +192,14 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       return readlane_i(tokbuf, pos - chunk0);
     };
 
-    // preload positions 0..W (kernels.cu:139-145)
-    const int npre = L < W + 1 ? L : W + 1;
-    for (int p = 0; p < npre; p++) {
-      const int tok = readlane_i(tokbuf, p);
-      ring_tok = (lane == p) ? tok : ring_tok;
-      load_row(p, tok);
+    // preload positions plo..c0+W (kernels.cu:139-145; 0..W for a whole sentence); the
+    // ring slot of position p is p mod R
+    const int pend = c0 + W + 1 < phi ? c0 + W + 1 : phi;
+    for (int p = plo; p < pend; p++) {
+      const int tok = readlane_i(tokbuf, p - chunk0);
+      const int sl = p % R;
+      ring_tok = (lane == sl) ? tok : ring_tok;
+      load_row(sl, tok);
     }
     float nT[G][E], nE[E];  // prefetched next group's target rows / next entering row
     int ntk[G];
@@ -195,20 +208,20 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       for (int k = 0; k < G; k++) {
         ntk[k] = 0;
         if (k < NT) {
-          ntk[k] = target_of(0, 0, k);
+          ntk[k] = target_of(c0, c0 % R, k);
           fetch(a.syn1, rd1, ntk[k], nT[k]);
         }
       }
     }
 
-    int cs = 0;  // ring slot of the centre
-    for (int w = 0; w < L; w++) {
+    int cs = c0 % R;  // ring slot of the centre
+    for (int w = c0; w < c1; w++) {
       const int lo = w - W < 0 ? 0 : w - W;
       const int hi = w + W > L - 1 ? L - 1 : w + W;
       const int nctx = hi - lo;  // window positions other than the centre
       const int enter_pos = w + W + 1;
       int enter_tok = 0;
-      if (enter_pos < L) {
+      if (enter_pos < phi) {
         enter_tok = entering_token(enter_pos);
         if (pipe) fetch(a.syn0, rd0, enter_tok, nE);
       }
@@ -239,7 +252,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
             nslot = cs + 1 == R ? 0 : cs + 1;
             ng0 = 0;
           }
-          if (nw < L) {
+          if (nw < c1) {
             next_centre(nw);
 #pragma unroll
             for (int k = 0; k < G; k++) {
@@ -347,12 +360,12 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
         // takes the slot before the flush atomics are issued
         float lv[E];
         int ltok = 0;
-        if (leave >= 0) {
+        if (leave >= plo) {
           ltok = readlane_i(ring_tok, slot);
 #pragma unroll
           for (int e = 0; e < E; e++) lv[e] = Ds[(size_t)slot * P + e * 64 + lane];
         }
-        if (enter < L) {
+        if (enter < phi) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           float* u0 = U0s + (size_t)slot * P;
           float* dd = Ds + (size_t)slot * P;
@@ -371,7 +384,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
           int rel = lane - cs;
           if (rel < 0) rel += R;
           if (rel > W) rel -= R;
-          const bool hot = lane < R && rel != -W && w + rel >= 0 && w + rel < L && ring_tok < a.hot_rows;
+          const bool hot = lane < R && rel != -W && w + rel >= plo && w + rel < phi && ring_tok < a.hot_rows;
           uint64_t hm = __ballot(hot);
           while (hm) {  // up to HB rows per batch: their reloads are in flight together
             constexpr int HB = E <= 4 ? 4 : 2;
@@ -414,7 +427,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
             }
           }
         }
-        if (leave >= 0) {
+        if (leave >= plo) {
           float* dst = a.out0 + (size_t)ltok * P;
 #pragma unroll
           for (int e = 0; e < E; e++) {
@@ -423,11 +436,11 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
           }
         }
       } else {
-        if (leave >= 0) {
+        if (leave >= plo) {
           flush_row(slot, readlane_i(ring_tok, slot));
           if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
         }
-        if (enter < L) {
+        if (enter < phi) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           load_row(slot, enter_tok);
         }
@@ -435,7 +448,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       cs = (cs + 1 == R) ? 0 : cs + 1;
     }
     // final flush of the contexts still in the window (kernels.cu:245-249)
-    for (int p = (L - W > 0 ? L - W : 0); p < L; p++) {
+    for (int p = (c1 - W > plo ? c1 - W : plo); p < phi; p++) {
       const int slot = p % R;
       flush_row(slot, readlane_i(ring_tok, slot));
     }

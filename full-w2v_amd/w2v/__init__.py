@@ -30,7 +30,7 @@ EXPORTS = [
     "w2v_kernel_time", "w2v_init_model", "w2v_unigram_table", "w2v_subsample_probs",
     "w2v_synth_corpus", "w2v_synth_cluster_of", "w2v_corpus_info", "w2v_corpus_download",
     "w2v_corpus_counts", "w2v_graph_ba_edges", "w2v_graph_ba", "w2v_synth_walks",
-    "w2v_autotune", "w2v_set_launch_shape", "w2v_get_launch_shape", "w2v_set_hot_rows", "w2v_auto_hot_rows", "w2v_set_coalesce",
+    "w2v_autotune", "w2v_set_launch_shape", "w2v_get_launch_shape", "w2v_set_hot_rows", "w2v_set_piece_length", "w2v_auto_hot_rows", "w2v_set_coalesce",
 ]
 
 
@@ -129,6 +129,7 @@ def lib() -> ctypes.CDLL:
             "w2v_autotune": [vp, i32, i64, i64, i64, ctypes.c_char_p, ctypes.POINTER(Tuning)],
             "w2v_set_launch_shape": [vp, i32, i32],
             "w2v_set_hot_rows": [vp, i64],
+            "w2v_set_piece_length": [vp, i32],
             "w2v_set_coalesce": [vp, i64, i64],
             "w2v_auto_hot_rows": [vp, vp, ctypes.c_float, u64, ctypes.POINTER(i64)],
             "w2v_get_launch_shape": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)],
@@ -395,6 +396,9 @@ class Trainer:
     def set_coalesce(self, max_sentences, max_tokens=0):
         """Queue host batches per stream and launch them together (0 sentences: off)."""
         self._chk(lib().w2v_set_coalesce(self.h, int(max_sentences), int(max_tokens)))
+
+    def set_piece_length(self, n):
+        self._chk(lib().w2v_set_piece_length(self.h, int(n)))
 
     def set_hot_rows(self, n):
         """Hogwild: context rows of token ids < n are written back and reloaded after

@@ -231,6 +231,15 @@ int w2v_set_launch_shape(w2v_ctx* ctx, int32_t waves_per_block, int32_t blocks);
  * Default W2V_DEFAULT_HOT_ROWS. No counterpart in the reference. */
 #define W2V_DEFAULT_HOT_ROWS 1024
 int w2v_set_hot_rows(w2v_ctx* ctx, int64_t n);
+/* Sentence pieces (device-sampling runs: w2v_train_device, w2v_train_epoch_graph,
+ * w2v_train_chunk): a sentence of more than n kept tokens is trained as pieces of n
+ * centres, each a work item of its own, so a launch with few long sentences does not end
+ * in a tail of single waves. A piece takes its windows' context rows from the whole
+ * sentence; rows near a piece boundary are then held by two waves at once, as rows of
+ * the same word in different sentences are in any Hogwild run. With one wave
+ * (deterministic mode) the pieces of a sentence run in order. n = 0 (the default) trains
+ * whole sentences, as the reference's one block per sentence (FULL-W2V-kernels.cu:106-250). */
+int w2v_set_piece_length(w2v_ctx* ctx, int32_t n);
 /* Hot-row count from the word counts (as given to w2v_subsample_probs; sample <= 0 means
  * no subsampling) and the current launch shape: word i is hot when
  *   (sentences in flight) x (2W+1) x (its share of the tokens kept by subsampling)
