@@ -1148,17 +1148,20 @@ int w2v_autotune(w2v_ctx* c, int32_t epoch, int64_t sb, int64_t se, int64_t max_
       char line[512];
       const size_t kl = strlen(key);
       while (fgets(line, sizeof(line), f)) {
-        int w = 0, b = 0;
+        int w = 0, b = 0, pl = 0;
         double wps = 0.0;
         if (strncmp(line, key, kl) == 0 && line[kl] == '|' &&
-            sscanf(line + kl + 1, "%d %d %lf", &w, &b, &wps) == 3 && w >= 1 && w <= 4 && b >= 1) {
+            sscanf(line + kl + 1, "%d %d %lf %d", &w, &b, &wps, &pl) >= 3 && w >= 1 && w <= 4 && b >= 1 &&
+            pl >= 0) {
           fclose(f);
+          if ((rc = w2v_set_piece_length(c, pl))) return rc;
           c->user_wpb = w;
           c->user_blocks = b;
           out->waves_per_block = w;
           out->blocks = b;
           out->words_per_sec = wps;
           out->cached = 1;
+          out->piece_length = pl;
           return W2V_OK;
         }
       }
@@ -1186,8 +1189,8 @@ int w2v_autotune(w2v_ctx* c, int32_t epoch, int64_t sb, int64_t se, int64_t max_
   if (e == hipSuccess) e = hipMemcpyAsync(k1, m1, rows, hipMemcpyDeviceToDevice, s.st);
   if (e == hipSuccess) e = hipMemcpyAsync(cnt, c->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, s.st);
   if (e == hipSuccess) e = hipStreamSynchronize(s.st);
-  const int uw0 = c->user_wpb, ub0 = c->user_blocks;
-  int best_w = 0, best_b = 0, tried = 0;
+  const int uw0 = c->user_wpb, ub0 = c->user_blocks, pl0 = c->piece_len;
+  int best_w = 0, best_b = 0, best_pl = pl0, tried = 0;
   double best = 0.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&e0);
@@ -1215,6 +1218,29 @@ int w2v_autotune(w2v_ctx* c, int32_t epoch, int64_t sb, int64_t se, int64_t max_
       }
     }
   }
+  // sentence piece lengths at the fastest shape
+  const int piece_cand[] = {0, 256, 128, 64};
+  for (int i = 0; i < 4 && tried > 0 && e == hipSuccess && rc == W2V_OK; i++) {
+    if (piece_cand[i] == pl0) continue;  // timed above
+    c->piece_len = piece_cand[i];
+    if ((rc = ensure_piece_buffers(c, s, n, se - sb))) break;
+    e = hipMemcpyAsync(m0, k0, rows, hipMemcpyDeviceToDevice, s.st);
+    if (e == hipSuccess) e = hipMemcpyAsync(m1, k1, rows, hipMemcpyDeviceToDevice, s.st);
+    if (e != hipSuccess) break;
+    c->user_wpb = best_w;
+    c->user_blocks = best_b;
+    if ((e = hipEventRecord(e0, s.st)) != hipSuccess) break;
+    if ((rc = enqueue_device_step(c, s, epoch, sb, se, false))) break;
+    if ((e = hipEventRecord(e1, s.st)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess) break;
+    float ms = 0.0f;
+    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+    const double wps = (double)n / (ms * 1e-3);
+    tried++;
+    if (wps > best) {
+      best = wps;
+      best_pl = piece_cand[i];
+    }
+  }
   // restore whatever happened above
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -1230,19 +1256,22 @@ int w2v_autotune(w2v_ctx* c, int32_t epoch, int64_t sb, int64_t se, int64_t max_
   s.ms = sms0;
   c->user_wpb = uw0;
   c->user_blocks = ub0;
+  c->piece_len = pl0;
   if (rc) return rc;
   if (e != hipSuccess || e2 != hipSuccess)
     return fail(c, W2V_ERR_HIP, "launch-shape tuning failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
   if (!tried) return fail(c, W2V_ERR_ARG, "no launch shape fits max_in_flight = %lld", (long long)max_in_flight);
   c->user_wpb = best_w;
   c->user_blocks = best_b;
+  c->piece_len = best_pl;
   out->waves_per_block = best_w;
   out->blocks = best_b;
   out->words_per_sec = best;
   out->candidates = tried;
+  out->piece_length = best_pl;
   if (cache_path) {
     if (FILE* f = fopen(cache_path, "a")) {
-      fprintf(f, "%s|%d %d %.6e\n", key, best_w, best_b, best);
+      fprintf(f, "%s|%d %d %.6e %d\n", key, best_w, best_b, best, best_pl);
       fclose(f);
     }
   }

@@ -48,7 +48,7 @@ class Params(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("waves_per_block", ctypes.c_int32), ("blocks", ctypes.c_int32),
                 ("words_per_sec", ctypes.c_double), ("candidates", ctypes.c_int32),
-                ("cached", ctypes.c_int32)]
+                ("cached", ctypes.c_int32), ("piece_length", ctypes.c_int32)]
 
 
 class Walks(ctypes.Structure):
@@ -382,13 +382,14 @@ class Trainer:
         return c
 
     def autotune(self, epoch, sent_begin, sent_end, max_in_flight=0, cache_path=None):
-        """Times the launch shapes on a sentence range (model restored), applies the
-        fastest; returns (waves_per_block, blocks, words_per_sec, candidates, cached)."""
+        """Times the launch shapes, then the sentence piece lengths, on a sentence range
+        (model restored) and applies the fastest; returns (waves_per_block, blocks,
+        words_per_sec, candidates, cached, piece_length)."""
         t = Tuning()
         cp = None if cache_path is None else os.fsencode(cache_path)
         self._chk(lib().w2v_autotune(self.h, epoch, int(sent_begin), int(sent_end), int(max_in_flight), cp,
                                      ctypes.byref(t)))
-        return t.waves_per_block, t.blocks, t.words_per_sec, t.candidates, bool(t.cached)
+        return t.waves_per_block, t.blocks, t.words_per_sec, t.candidates, bool(t.cached), t.piece_length
 
     def set_launch_shape(self, waves_per_block=0, blocks=0):
         self._chk(lib().w2v_set_launch_shape(self.h, int(waves_per_block), int(blocks)))

@@ -259,15 +259,19 @@ int w2v_auto_hot_rows(w2v_ctx* ctx, const uint64_t* counts, float sample, uint64
  * corpora), by training sentences [sent_begin, sent_end) of the device corpus in epoch
  * `epoch` from a snapshot of the model; the model and the counters are restored after
  * each trial, so tuning changes no training state. The fastest shape is applied as by
- * w2v_set_launch_shape and returned. With cache_path, a line keyed by (device, kernel,
- * storage, d, W, K, cap) is reused when present and appended otherwise. Hogwild runs
- * only. No counterpart in the reference, whose shape is fixed by -kernel-batch-size. */
+ * w2v_set_launch_shape and returned. Then, at that shape, the sentence piece lengths 0
+ * (whole sentences), 256, 128 and 64 are timed the same way and the fastest is applied as
+ * by w2v_set_piece_length (this is how many waves share a long sentence). With cache_path,
+ * a line keyed by (device, kernel, storage, d, W, K, cap) is reused when present and
+ * appended otherwise. Hogwild runs only. No counterpart in the reference, whose shape is
+ * fixed by -kernel-batch-size and which trains one sentence per block. */
 typedef struct w2v_tuning {
   int32_t waves_per_block;
   int32_t blocks;
   double words_per_sec; /* of the chosen shape on the tuning range */
-  int32_t candidates;   /* shapes timed (0 when taken from the cache) */
+  int32_t candidates;   /* trials timed (0 when taken from the cache) */
   int32_t cached;
+  int32_t piece_length; /* chosen sentence piece length (w2v_set_piece_length; 0 = whole) */
 } w2v_tuning;
 int w2v_autotune(w2v_ctx* ctx, int32_t epoch, int64_t sent_begin, int64_t sent_end, int64_t max_in_flight,
                  const char* cache_path, w2v_tuning* out);

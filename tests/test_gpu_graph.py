@@ -103,18 +103,18 @@ def test_autotune_leaves_state_and_applies_fastest(w2vmod, tmp_path, kernel, sto
     before = tr.download()
     st0 = tr.stats()
     cache = str(tmp_path / "tune.txt")
-    w, b, wps, n, cached = tr.autotune(0, 0, ns // 2, cache_path=cache)
-    print(f"{kernel}: tuned {w} x {b} at {wps / 1e6:.1f} M words/s over {n} shapes", flush=True)
-    assert not cached and n >= 3 and wps > 0
+    w, b, wps, n, cached, pl = tr.autotune(0, 0, ns // 2, cache_path=cache)
+    print(f"{kernel}: tuned {w} x {b}, pieces {pl}, at {wps / 1e6:.1f} M words/s over {n} trials", flush=True)
+    assert not cached and n >= 3 + 3 and wps > 0 and pl in (0, 256, 128, 64)
     assert (w, b) == tr.launch_shape()
     after = tr.download()
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
     assert tr.stats() == st0
     # capped concurrency, and the cache on the next call
-    w2, b2, _, n2, _ = tr.autotune(0, 0, ns // 2, max_in_flight=512)
+    w2, b2, _, n2, _, _ = tr.autotune(0, 0, ns // 2, max_in_flight=512)
     assert w2 * b2 <= 512 and n2 >= 1
-    w3, b3, _, n3, cached3 = tr.autotune(0, 0, ns // 2, cache_path=cache)
-    assert cached3 and n3 == 0 and (w3, b3) == (w, b) == tr.launch_shape()
+    w3, b3, _, n3, cached3, pl3 = tr.autotune(0, 0, ns // 2, cache_path=cache)
+    assert cached3 and n3 == 0 and (w3, b3) == (w, b) == tr.launch_shape() and pl3 == pl
     # deterministic contexts are not tuned
     det, _ = make(w2vmod, kernel, storage, "deterministic", n_sent=100)
     with pytest.raises(w2vmod.W2VError):

@@ -143,3 +143,38 @@ def test_hogwild_pieces_heldout_loss(w2vmod, kernel, storage):
         assert st["sentences"] == ns
     print(f"{kernel}: held-out loss whole {losses[0]:.4f}, pieces of 64 {losses[64]:.4f}", flush=True)
     assert losses[64] <= 1.01 * losses[0]
+
+
+def test_pieces_in_captured_epoch_equal_eager(orc, w2vmod):
+    """The piece list is built inside the captured epoch graph: deterministic mode, pieces
+    of 5, two steps, graph launch (and a replay for the next epoch) bit-identical to the
+    eager steps."""
+    V, d, W, K, seed = 500, 64, 2, 3, 21
+    rng = np.random.default_rng(10)
+    syn0 = ((rng.random((V, d)) - 0.5) / d).astype(np.float32)
+    syn1 = (rng.standard_normal((V, d)) * 0.5 / np.sqrt(d)).astype(np.float32)
+    lens = [33, 2, 17, 40, 9, 26]
+    tok, off = corpus(V, lens, rng)
+    cn = np.bincount(tok, minlength=V).astype(np.uint64) + 1
+    T = int(off[-1])
+    subs = orc.subsample_probs(cn, 0.05, T)
+    uni = orc.unigram_table(cn, 10007)
+    bounds = np.array([0, 3, len(lens)], np.int64)
+    out = []
+    for graph in (False, True):
+        tr = w2vmod.Trainer(V, d, W, K, msl=64, sigmoid="exact", mode="deterministic", seed=seed)
+        tr.upload(syn0, syn1)
+        tr.tables_upload(uni, subs)
+        tr.set_schedule(0.025, 2, T)
+        tr.corpus_upload(tok, off)
+        tr.set_piece_length(5)
+        for epoch in (0, 1):
+            if graph:
+                tr.train_epoch_graph(epoch, bounds)
+            else:
+                for i in range(len(bounds) - 1):
+                    tr.train_device(epoch, int(bounds[i]), int(bounds[i + 1]))
+            tr.sync()
+        out.append(tr.download())
+        tr.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
