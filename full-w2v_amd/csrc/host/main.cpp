@@ -45,6 +45,7 @@ struct Options {
   std::string precision = "fp32", save_ckpt, resume;
   int device_sampling = 0, device = 0, exact_sigmoid = 0;
   long long hot_rows = -1;  // -1: from the vocabulary counts (w2v_auto_hot_rows)
+  int piece_length = 0;     // w2v_set_piece_length (device sampling)
   int ingest_threads = 0;   // vocabulary and corpus passes over the train file; 0: all cores
   unsigned long long seed = 1;
 };
@@ -90,6 +91,8 @@ void usage(const char* prog) {
   printf("\t-exact-sigmoid <0|1>\n\t\tUse 1/(1+exp(-x)) instead of the 1000-entry table\n");
   printf("\t-hot-rows <int>\n\t\tContext rows of the n most frequent words are written back and reloaded\n"
          "\t\tafter every centre (-1: chosen from the word counts; 0: off)\n");
+  printf("\t-piece-length <int>\n\t\tWith -device-sampling 1, train sentences longer than n kept tokens as\n"
+         "\t\tpieces of n centres on separate waves (0: whole sentences, the default)\n");
   printf("\t-ingest-threads <int>\n\t\tThreads for the vocabulary and corpus passes over -train (0: all cores;\n"
          "\t\tthe result does not depend on it)\n");
   printf("\t-device <int>\n\t-seed <int>\n");
@@ -163,6 +166,7 @@ Options parse(int argc, char** argv) {
   num("-device", o.device);
   if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = strtoull(argv[i + 1], nullptr, 10);
   if ((i = arg_pos("-hot-rows", argc, argv)) > 0 && i + 1 < argc) o.hot_rows = strtoll(argv[i + 1], nullptr, 10);
+  if ((i = arg_pos("-piece-length", argc, argv)) > 0 && i + 1 < argc) o.piece_length = atoi(argv[i + 1]);
   num("-ingest-threads", o.ingest_threads);
   if (o.debug > 0) {
     fprintf(stderr, "** FILES & I/O **\n");
@@ -523,6 +527,7 @@ int main(int argc, char** argv) {
   } else if ((rc = w2v_set_hot_rows(ctx, o.hot_rows))) {
     die("w2v_set_hot_rows", ctx, rc);
   }
+  if ((rc = w2v_set_piece_length(ctx, o.piece_length))) die("w2v_set_piece_length", ctx, rc);
   if (o.pre_shuffle) corpus.shuffle();
   if (o.debug > 0) fprintf(stderr, "Network ready\n");
 

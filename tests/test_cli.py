@@ -136,6 +136,7 @@ def parse_contract(out):
     ["-precision", "bf16", "-device-sampling", "1"],
     ["-precision", "bf16", "-device-sampling", "1", "-hot-rows", "0"],
     ["-precision", "fp32", "-hot-rows", "50"],
+    ["-precision", "bf16", "-device-sampling", "1", "-piece-length", "8"],
 ])
 def test_cli_trains_and_exports(tmp_path, extra):
     words_text(tmp_path)
