@@ -166,9 +166,6 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       L = a.length[s];
     }
     if (L <= c0) continue;
-    // centres [c0, c1); context positions [plo, phi) (the windows of those centres)
-    const int c1 = a.pieces && c0 + a.piece_len < L ? c0 + a.piece_len : L;
-    const int plo = c0 - W > 0 ? c0 - W : 0, phi = c1 + W < L ? c1 + W : L;
     float alpha;
     int64_t raw = 0;
     if (a.alpha) {
@@ -179,16 +176,31 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       alpha = a.alpha0 * (1.0f - (float)wca / (float)a.total_words);
       if (alpha < a.alpha0 * 0.0001f) alpha = a.alpha0 * 0.0001f;
     }
-    words += (unsigned long long)(c1 - c0);
     sents += c0 == 0 ? 1 : 0;
+    // A piece (centres [c0, c1) of the sentence) is trained as a sentence of its own over
+    // positions [c0 - W, c1 + W) clipped to the sentence, with only [c0, c1) as centres:
+    // every window of those centres lies inside that span, so clipping windows to it is
+    // clipping them to the sentence. Whole sentences: plo = 0, c1 = L.
+    int c1 = L;
+    if (a.pieces) {
+      c1 = c0 + a.piece_len < L ? c0 + a.piece_len : L;
+      const int plo = c0 - W > 0 ? c0 - W : 0;
+      L = (c1 + W < L ? c1 + W : L) - plo;
+      base += plo;
+      raw += plo;  // (negatives are keyed by corpus position; alpha is the sentence's)
+      c0 -= plo;
+      c1 -= plo;
+    }
+    __builtin_assume(c0 >= 0 && c0 <= W && c0 < c1 && c1 <= L);
+    words += (unsigned long long)(c1 - c0);
     uint32_t rng = (uint32_t)key2(a.mixed_seed, 0x5352ull ^ ((uint64_t)a.epoch << 40), (uint64_t)(item * 64 + lane)) | 1u;
 
     // Token ids and negatives are staged 64 lanes at a time, with the following batch
     // loaded one batch ahead: the registers read in the centre loop are only ever
     // written by moves, so no wait on outstanding memory operations is needed there.
-    int chunk0 = plo;
-    int tokbuf = (plo + lane < L) ? a.tokens[base + plo + lane] : 0;
-    int toknext = (plo + 64 + lane < L) ? a.tokens[base + plo + 64 + lane] : 0;
+    int chunk0 = 0;
+    int tokbuf = (lane < L) ? a.tokens[base + lane] : 0;
+    int toknext = (64 + lane < L) ? a.tokens[base + 64 + lane] : 0;
     int ring_tok = 0;
     auto neg_batch = [&](int nw) -> int {  // negatives of centres nw .. nw+cpb-1
       const int cw = nw + lane / K, kk = lane % K;
@@ -302,13 +314,12 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
     };
     auto negative = [&](int w, int k) -> int { return readlane_i(negbuf, (w - nw0) * K + k); };
 
-    // preload positions plo..c0+W (0..W for a whole sentence); ring slot = position mod R
-    const int pend = c0 + W + 1 < phi ? c0 + W + 1 : phi;
-    for (int p = plo; p < pend; p++) {
-      const int tok = readlane_i(tokbuf, p - chunk0);
-      const int sl = p % R;
-      ring_tok = (lane == sl) ? tok : ring_tok;
-      load_row(sl, tok);
+    // preload positions 0..c0+W (c0 <= W, so slot = position)
+    const int npre = c0 + W + 1 < L ? c0 + W + 1 : L;
+    for (int p = 0; p < npre; p++) {
+      const int tok = readlane_i(tokbuf, p);
+      ring_tok = (lane == p) ? tok : ring_tok;
+      load_row(p, tok);
     }
     // Hogwild mode loads the next centre's target rows and the next entering context
     // row one centre ahead (packed bf16 pairs); deterministic mode loads in order.
@@ -338,17 +349,17 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       }
     };
     if (pipe) {
-      targets_of(c0, c0 % R, ntk);
+      targets_of(c0, c0, ntk);
 #pragma unroll
       for (int k = 0; k < NTB; k++)
         if (k < NT) fetch_pairs(a.bsyn1, ntk[k], nT[k]);
     }
 
-    int cs = c0 % R;
+    int cs = c0;
     for (int w = c0; w < c1; w++) {
       const int enter_pos = w + W + 1;
       int enter_tok = 0;
-      if (enter_pos < phi) {
+      if (enter_pos < L) {
         enter_tok = entering_token(enter_pos);
         if (pipe) fetch_pairs(a.bsyn0, enter_tok, nE);
       }
@@ -447,7 +458,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
         int rel = lane - cs;
         if (rel < 0) rel += R;
         if (rel > W) rel -= R;
-        const bool hot = lane < R && rel != -W && w + rel >= plo && w + rel < phi && ring_tok < a.hot_rows;
+        const bool hot = lane < R && rel != -W && w + rel >= 0 && w + rel < L && ring_tok < a.hot_rows;
         const uint64_t hm = __ballot(hot);
         if (hm) {
           hot_any = true;
@@ -544,7 +555,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       uint32_t lp[NP];
       int ltok = 0;
       if (pipe) {
-        if (leave >= plo) {
+        if (leave >= 0) {
           ltok = readlane_i(ring_tok, slot);
           const float* uf = Uf + slot * ly.PU;
           u0_get(slot, lp);
@@ -555,7 +566,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
             lv[e] = make_float2(u.x - bf_lo(lp[e]), u.y - bf_hi(lp[e]));
           }
         }
-        if (enter < phi) {
+        if (enter < L) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           float* uf = Uf + slot * ly.PU;
           u0_put(slot, nE);
@@ -619,7 +630,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
 
       // ---- ring advance: flush the leaving context, load the entering one
       if (pipe) {
-        if (leave >= plo) {
+        if (leave >= 0) {
 #pragma unroll
           for (int e = 0; e < NP; e++) {
             const int j = 2 * lane + 128 * e;
@@ -628,11 +639,11 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
           }
         }
       } else {
-        if (leave >= plo) {
+        if (leave >= 0) {
           flush_row(slot, readlane_i(ring_tok, slot));
           if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
         }
-        if (enter < phi) {
+        if (enter < L) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           load_row(slot, enter_tok);
         }
@@ -641,7 +652,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
       __builtin_amdgcn_wave_barrier();
       cs = (cs + 1 == R) ? 0 : cs + 1;
     }
-    for (int p = (c1 - W > plo ? c1 - W : plo); p < phi; p++) {
+    for (int p = (c1 - W > 0 ? c1 - W : 0); p < L; p++) {
       const int slot = p % R;
       flush_row(slot, readlane_i(ring_tok, slot));
     }

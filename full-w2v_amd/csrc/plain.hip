@@ -90,9 +90,6 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       L = a.length[s];
     }
     if (L <= c0) continue;
-    // centres [c0, c1); context positions [plo, phi) (the windows of those centres)
-    const int c1 = a.pieces && c0 + a.piece_len < L ? c0 + a.piece_len : L;
-    const int plo = c0 - W > 0 ? c0 - W : 0, phi = c1 + W < L ? c1 + W : L;
     float alpha;
     int64_t raw = 0;
     if (a.alpha) {
@@ -104,15 +101,30 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       alpha = a.alpha0 * (1.0f - (float)wca / (float)a.total_words);
       if (alpha < a.alpha0 * 0.0001f) alpha = a.alpha0 * 0.0001f;
     }
-    words += (unsigned long long)(c1 - c0);
     sents += c0 == 0 ? 1 : 0;
+    // A piece (centres [c0, c1) of the sentence) is trained as a sentence of its own over
+    // positions [c0 - W, c1 + W) clipped to the sentence, with only [c0, c1) as centres:
+    // every window of those centres lies inside that span, so clipping windows to it is
+    // clipping them to the sentence. Whole sentences: plo = 0, c1 = L.
+    int c1 = L;
+    if (a.pieces) {
+      c1 = c0 + a.piece_len < L ? c0 + a.piece_len : L;
+      const int plo = c0 - W > 0 ? c0 - W : 0;
+      L = (c1 + W < L ? c1 + W : L) - plo;
+      base += plo;
+      raw += plo;  // (negatives are keyed by corpus position; alpha is the sentence's)
+      c0 -= plo;
+      c1 -= plo;
+    }
+    __builtin_assume(c0 >= 0 && c0 <= W && c0 < c1 && c1 <= L);
+    words += (unsigned long long)(c1 - c0);
 
     // Token ids and negatives are staged 64 lanes at a time, with the following batch
     // loaded one batch ahead: the registers read in the centre loop are only ever
     // written by moves, so no wait on outstanding memory operations is needed there.
-    int chunk0 = plo;
-    int tokbuf = (plo + lane < L) ? a.tokens[base + plo + lane] : 0;
-    int toknext = (plo + 64 + lane < L) ? a.tokens[base + plo + 64 + lane] : 0;
+    int chunk0 = 0;
+    int tokbuf = (lane < L) ? a.tokens[base + lane] : 0;
+    int toknext = (64 + lane < L) ? a.tokens[base + 64 + lane] : 0;
     int ring_tok = 0;  // lane r holds the token id of ring slot r
     auto neg_batch = [&](int nw) -> int {  // negatives of centres nw .. nw+cpb-1
       const int cw = nw + lane / K, kk = lane % K;
@@ -192,14 +204,12 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       return readlane_i(tokbuf, pos - chunk0);
     };
 
-    // preload positions plo..c0+W (kernels.cu:139-145; 0..W for a whole sentence); the
-    // ring slot of position p is p mod R
-    const int pend = c0 + W + 1 < phi ? c0 + W + 1 : phi;
-    for (int p = plo; p < pend; p++) {
-      const int tok = readlane_i(tokbuf, p - chunk0);
-      const int sl = p % R;
-      ring_tok = (lane == sl) ? tok : ring_tok;
-      load_row(sl, tok);
+    // preload positions 0..c0+W (kernels.cu:139-145; c0 <= W, so slot = position)
+    const int npre = c0 + W + 1 < L ? c0 + W + 1 : L;
+    for (int p = 0; p < npre; p++) {
+      const int tok = readlane_i(tokbuf, p);
+      ring_tok = (lane == p) ? tok : ring_tok;
+      load_row(p, tok);
     }
     float nT[G][E], nE[E];  // prefetched next group's target rows / next entering row
     int ntk[G];
@@ -208,20 +218,20 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       for (int k = 0; k < G; k++) {
         ntk[k] = 0;
         if (k < NT) {
-          ntk[k] = target_of(c0, c0 % R, k);
+          ntk[k] = target_of(c0, c0, k);
           fetch(a.syn1, rd1, ntk[k], nT[k]);
         }
       }
     }
 
-    int cs = c0 % R;  // ring slot of the centre
+    int cs = c0;  // ring slot of the centre
     for (int w = c0; w < c1; w++) {
       const int lo = w - W < 0 ? 0 : w - W;
       const int hi = w + W > L - 1 ? L - 1 : w + W;
       const int nctx = hi - lo;  // window positions other than the centre
       const int enter_pos = w + W + 1;
       int enter_tok = 0;
-      if (enter_pos < phi) {
+      if (enter_pos < L) {
         enter_tok = entering_token(enter_pos);
         if (pipe) fetch(a.syn0, rd0, enter_tok, nE);
       }
@@ -360,12 +370,12 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
         // takes the slot before the flush atomics are issued
         float lv[E];
         int ltok = 0;
-        if (leave >= plo) {
+        if (leave >= 0) {
           ltok = readlane_i(ring_tok, slot);
 #pragma unroll
           for (int e = 0; e < E; e++) lv[e] = Ds[(size_t)slot * P + e * 64 + lane];
         }
-        if (enter < phi) {
+        if (enter < L) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           float* u0 = U0s + (size_t)slot * P;
           float* dd = Ds + (size_t)slot * P;
@@ -384,7 +394,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
           int rel = lane - cs;
           if (rel < 0) rel += R;
           if (rel > W) rel -= R;
-          const bool hot = lane < R && rel != -W && w + rel >= plo && w + rel < phi && ring_tok < a.hot_rows;
+          const bool hot = lane < R && rel != -W && w + rel >= 0 && w + rel < L && ring_tok < a.hot_rows;
           uint64_t hm = __ballot(hot);
           while (hm) {  // up to HB rows per batch: their reloads are in flight together
             constexpr int HB = E <= 4 ? 4 : 2;
@@ -427,7 +437,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
             }
           }
         }
-        if (leave >= plo) {
+        if (leave >= 0) {
           float* dst = a.out0 + (size_t)ltok * P;
 #pragma unroll
           for (int e = 0; e < E; e++) {
@@ -436,11 +446,11 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
           }
         }
       } else {
-        if (leave >= plo) {
+        if (leave >= 0) {
           flush_row(slot, readlane_i(ring_tok, slot));
           if (fence) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
         }
-        if (enter < phi) {
+        if (enter < L) {
           ring_tok = (lane == slot) ? enter_tok : ring_tok;
           load_row(slot, enter_tok);
         }
@@ -448,7 +458,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
       cs = (cs + 1 == R) ? 0 : cs + 1;
     }
     // final flush of the contexts still in the window (kernels.cu:245-249)
-    for (int p = (c1 - W > plo ? c1 - W : plo); p < phi; p++) {
+    for (int p = (c1 - W > 0 ? c1 - W : 0); p < L; p++) {
       const int slot = p % R;
       flush_row(slot, readlane_i(ring_tok, slot));
     }
