@@ -35,13 +35,13 @@ MfmaLayout mfma_layout(int W, int K, int P, bool delta, bool hw) {
 }
 
 
-static const void* mfma_select(int P, int NT, int R, bool hw) {
+static const void* mfma_select(int P, int NT, int R, bool hw, bool pc) {
   const int np = (P + 127) / 128, ntb = nt_bucket(NT), mt = R > 16 ? 2 : 1;
   switch (np) {
-    case 1: return mfma_instance_np1(ntb, mt, hw);
-    case 2: return mfma_instance_np2(ntb, mt, hw);
-    case 3: return mfma_instance_np3(ntb, mt, hw);
-    case 4: return mfma_instance_np4(ntb, mt, hw);
+    case 1: return mfma_instance_np1(ntb, mt, hw, pc);
+    case 2: return mfma_instance_np2(ntb, mt, hw, pc);
+    case 3: return mfma_instance_np3(ntb, mt, hw, pc);
+    case 4: return mfma_instance_np4(ntb, mt, hw, pc);
     default: return nullptr;
   }
 }
@@ -49,7 +49,7 @@ static const void* mfma_select(int P, int NT, int R, bool hw) {
 bool mfma_supported(int d, int W, int K) { return d <= 512 && 2 * W + 1 <= 32 && K + 1 <= 16; }
 
 hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hipStream_t st) {
-  const void* fn = mfma_select(a.P, a.K + 1, 2 * a.W + 1, a.rd0 == nullptr && !a.fence);
+  const void* fn = mfma_select(a.P, a.K + 1, 2 * a.W + 1, a.rd0 == nullptr && !a.fence, a.pieces != nullptr);
   if (!fn) return hipErrorInvalidValue;
   const MfmaLayout ly = mfma_layout(a.W, a.K, a.P, a.rd0 != nullptr, a.rd0 == nullptr && !a.fence);
   const size_t lds = (size_t)ly.wave_bytes * waves_per_block;
@@ -63,7 +63,7 @@ hipError_t launch_mfma(const UpdateArgs& a, int waves_per_block, int blocks, hip
 
 hipError_t occupancy_mfma(int P, int K, int W, bool delta, int waves_per_block, size_t lds,
                           int* blocks_per_cu) {
-  const void* fn = mfma_select(P, K + 1, 2 * W + 1, !delta);
+  const void* fn = mfma_select(P, K + 1, 2 * W + 1, !delta, false);
   if (!fn) return hipErrorInvalidValue;
   hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;

@@ -118,7 +118,9 @@ __device__ __forceinline__ float tile_grad(float f, bool positive, float alpha, 
 // Register state grows with NP x NTB; up to 18 it fits the 256 registers that let two
 // waves share a SIMD, which the launch bound holds the compiler to (larger shapes would
 // spill under it and run one wave per SIMD instead).
-template <int NP, int NTB, int MT, bool HW>
+// PC: work items may be sentence pieces (UpdateArgs::pieces). The Hogwild instantiation
+// has a variant without, whose centre loop compiles as for whole sentences only.
+template <int NP, int NTB, int MT, bool HW, bool PC = true>
 __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
     sgns_mfma_kernel(UpdateArgs a, MfmaLayout ly) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -142,7 +144,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
   typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
   u32x16_t u0v[NP];  // load-time ring pairs when held in registers (element = ring slot)
   u32x8_t u0w[MT == 2 ? NP : 1];  // slots 16..23 of a two-tile ring
-  const int64_t n_items = a.pieces ? *a.n_pieces : a.n_sent;
+  const int64_t n_items = PC && a.pieces ? *a.n_pieces : a.n_sent;
 
   for (;;) {
     unsigned s32 = 0;
@@ -151,7 +153,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
     if (item >= n_items) break;
     int64_t s = item;
     int c0 = 0;  // first centre of this work item
-    if (a.pieces) {
+    if (PC && a.pieces) {
       const int64_t pc = a.pieces[item];
       s = pc >> 32;
       c0 = (int)(pc & 0xFFFFFFFF);
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
     // every window of those centres lies inside that span, so clipping windows to it is
     // clipping them to the sentence. Whole sentences: plo = 0, c1 = L.
     int c1 = L;
-    if (a.pieces) {
+    if (PC && a.pieces) {
       c1 = c0 + a.piece_len < L ? c0 + a.piece_len : L;
       const int plo = c0 - W > 0 ? c0 - W : 0;
       L = (c1 + W < L ? c1 + W : L) - plo;
@@ -666,9 +668,10 @@ __global__ void __launch_bounds__(256, (NP * NTB <= 18 ? 2 : 1))
 }
 
 // Instantiations live in mfma_np<NP>.hip (one translation unit per NP).
-const void* mfma_instance_np1(int ntb, int mt, bool hw);
-const void* mfma_instance_np2(int ntb, int mt, bool hw);
-const void* mfma_instance_np3(int ntb, int mt, bool hw);
-const void* mfma_instance_np4(int ntb, int mt, bool hw);
+// (hw selects the Hogwild instantiation; pc its variant that takes sentence pieces)
+const void* mfma_instance_np1(int ntb, int mt, bool hw, bool pc);
+const void* mfma_instance_np2(int ntb, int mt, bool hw, bool pc);
+const void* mfma_instance_np3(int ntb, int mt, bool hw, bool pc);
+const void* mfma_instance_np4(int ntb, int mt, bool hw, bool pc);
 
 }  // namespace w2v

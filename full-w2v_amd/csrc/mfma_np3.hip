@@ -3,13 +3,15 @@
 
 namespace w2v {
 
-const void* mfma_instance_np3(int ntb, int mt, bool hw) {
+const void* mfma_instance_np3(int ntb, int mt, bool hw, bool pc) {
 #define W2V_NT(NTB)                                                                     \
   if (ntb == NTB) {                                                                     \
-    if (mt == 1) return hw ? (const void*)sgns_mfma_kernel<3, NTB, 1, true>            \
-                           : (const void*)sgns_mfma_kernel<3, NTB, 1, false>;          \
-    return hw ? (const void*)sgns_mfma_kernel<3, NTB, 2, true>                        \
-              : (const void*)sgns_mfma_kernel<3, NTB, 2, false>;                      \
+    if (mt == 1) return !hw ? (const void*)sgns_mfma_kernel<3, NTB, 1, false>         \
+                            : pc ? (const void*)sgns_mfma_kernel<3, NTB, 1, true, true> \
+                                 : (const void*)sgns_mfma_kernel<3, NTB, 1, true, false>; \
+    return !hw ? (const void*)sgns_mfma_kernel<3, NTB, 2, false>                      \
+               : pc ? (const void*)sgns_mfma_kernel<3, NTB, 2, true, true>            \
+                    : (const void*)sgns_mfma_kernel<3, NTB, 2, true, false>;          \
   }
   W2V_NT(2) W2V_NT(4) W2V_NT(6) W2V_NT(8) W2V_NT(12) W2V_NT(16)
 #undef W2V_NT
