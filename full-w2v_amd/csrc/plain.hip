@@ -50,7 +50,9 @@ constexpr int plain_group(int E) { return E <= 5 ? 6 : E <= 8 ? 4 : 2; }
 // are independent and overlap. Target k meets context c after target k-1 has updated
 // it and each target meets the contexts left to right, so every element sees the same
 // operations in the same order as the one-pair-at-a-time loop (kernels.cu:151-215).
-template <int E, bool HW>
+// PC: work items may be sentence pieces (UpdateArgs::pieces). The Hogwild instantiation
+// has a variant without, whose centre loop compiles as for whole sentences only.
+template <int E, bool HW, bool PC = true>
 __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
   constexpr int G = HW ? plain_group(E) : 1;
   extern __shared__ float lds[];
@@ -65,7 +67,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
   float* Ds = U0s + (size_t)R * P;
   const int cpb = K > 0 ? 64 / K : 0;  // centres whose negatives fit in one VGPR
   unsigned long long words = 0, sents = 0;
-  const int64_t n_items = a.pieces ? *a.n_pieces : a.n_sent;
+  const int64_t n_items = PC && a.pieces ? *a.n_pieces : a.n_sent;
 
   for (;;) {
     unsigned s32 = 0;
@@ -74,7 +76,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
     if (item >= n_items) break;
     int64_t s = item;
     int c0 = 0;  // first centre of this work item
-    if (a.pieces) {
+    if (PC && a.pieces) {
       const int64_t pc = a.pieces[item];
       s = pc >> 32;
       c0 = (int)(pc & 0xFFFFFFFF);
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
     // every window of those centres lies inside that span, so clipping windows to it is
     // clipping them to the sentence. Whole sentences: plo = 0, c1 = L.
     int c1 = L;
-    if (a.pieces) {
+    if (PC && a.pieces) {
       c1 = c0 + a.piece_len < L ? c0 + a.piece_len : L;
       const int plo = c0 - W > 0 ? c0 - W : 0;
       L = (c1 + W < L ? c1 + W : L) - plo;
@@ -474,14 +476,15 @@ __global__ void __launch_bounds__(256) sgns_plain_kernel(UpdateArgs a) {
 size_t plain_lds_per_wave(int W, int P) { return (size_t)2 * (2 * W + 1) * P * sizeof(float); }
 
 template <int E>
-static const void* plain_fn(bool hw) {
-  return hw ? (const void*)sgns_plain_kernel<E, true> : (const void*)sgns_plain_kernel<E, false>;
+static const void* plain_fn(bool hw, bool pc) {
+  if (!hw) return (const void*)sgns_plain_kernel<E, false>;
+  return pc ? (const void*)sgns_plain_kernel<E, true, true> : (const void*)sgns_plain_kernel<E, true, false>;
 }
 
 template <int E>
 static hipError_t launch_plain_e(const UpdateArgs& a, int waves_per_block, int blocks,
                                  hipStream_t st) {
-  const void* fn = plain_fn<E>(a.rd0 == nullptr && a.rd1 == nullptr && !a.fence);
+  const void* fn = plain_fn<E>(a.rd0 == nullptr && a.rd1 == nullptr && !a.fence, a.pieces != nullptr);
   const size_t lds = plain_lds_per_wave(a.W, a.P) * waves_per_block;
   hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
@@ -492,7 +495,7 @@ static hipError_t launch_plain_e(const UpdateArgs& a, int waves_per_block, int b
 
 template <int E>
 static hipError_t occupancy_plain_e(bool hw, int waves_per_block, size_t lds, int* blocks_per_cu) {
-  const void* fn = plain_fn<E>(hw);
+  const void* fn = plain_fn<E>(hw, false);
   hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err != hipSuccess) return err;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 64 * waves_per_block, lds);
